@@ -1,0 +1,395 @@
+// Whitening and linear-model GEMMs on the CDNA4 matrix cores.
+//
+// Y[m, n] = sum_k A[m, k] B[n, k]   (A: M x K row-major, B: N x K row-major)
+//
+//  * whitening: A = residuals R (chains x observations), B = W, the lower
+//    triangular whitener (|W r|^2 = r^T Cd^-1 r); the epilogue squares Y
+//    and reduces each row over the tile, so W r never reaches HBM.  Only the
+//    k <= n part of a lower-triangular B is visited.
+//  * linear forward: A = theta (chains x parameters), B = G (observations x
+//    parameters); the epilogue writes Y - d (the residual matrix) or, when B
+//    is the pre-whitened W G, reduces (Y - W d)^2 like the whitening.
+//
+// FP32 path: v_mfma_f32_32x32x2_f32.  Split-BF16 path: operands are stored
+// as hi + lo BF16 pairs and each tile accumulates hi*hi + hi*lo + lo*hi with
+// v_mfma_f32_32x32x16_bf16 in FP32 — one BF16 product plus one correction
+// (refinement) step; with plain=1 only hi*hi is formed.
+//
+// Tiling: 256 threads = 4 wavefronts in a 2 x 2 grid; block tile 128 x 128;
+// each wavefront owns 64 x 64 = 2 x 2 MFMA 32x32 tiles.  Operands are
+// staged through LDS, double-buffered, one barrier per K step.  Inside a K
+// step the lanes of the upper half-wave take the upper half of the K slice
+// (a permutation of the summation order), so every lane reads its A and B
+// fragments with 16-byte LDS loads.  Operands must be zero-padded to
+// multiples of the tile sizes (the host guarantees it).
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_bf16.h>
+
+namespace altar {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 128, BN = 128;
+
+// ---------------------------------------------------------------- epilogue
+// 32x32 MFMA accumulator layout: register v of lane l holds
+// row i = 8 (v / 4) + 4 (l / 32) + (v % 4), column j = l % 32.
+__device__ inline int acc_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+template <int EPI>
+__device__ inline void gemm_epilogue(floatx16 (&acc)[2][2], const GemmEpi &ep, int m0, int n0, int tile,
+                                     int wm, int wn, float *red /* [2][BM] */)
+{
+    const int lane = threadIdx.x & 63;
+    if constexpr (EPI == EPI_STORE) {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int col = n0 + wn * 64 + ni * 32 + (lane & 31);
+                const float bias = ep.bias ? ep.bias[col] : 0.0f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int row = m0 + wm * 64 + mi * 32 + acc_row(v, lane);
+                    ep.out[(size_t)row * ep.ldo + col] = acc[mi][ni][v] - bias;
+                }
+            }
+    } else {
+        // sum over this block's 128 columns of (Y - bias)^2, per row
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) {
+            float s[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) s[v] = 0.0f;
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int col = n0 + wn * 64 + ni * 32 + (lane & 31);
+                const float bias = (EPI == EPI_SUMSQ_BIAS) ? ep.bias[col] : 0.0f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    float y = acc[mi][ni][v] - bias;
+                    s[v] += y * y;
+                }
+            }
+            // reduce across the 32 lanes that share rows (xor within a half-wave)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+#pragma unroll
+                for (int off = 1; off < 32; off <<= 1) s[v] += __shfl_xor(s[v], off, 64);
+            }
+            if ((lane & 31) == 0) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) red[wn * BM + wm * 64 + mi * 32 + acc_row(v, lane)] = s[v];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < BM) {
+            const int row = m0 + threadIdx.x;
+            ep.partial[(size_t)row * ep.ntiles + tile] = red[threadIdx.x] + red[BM + threadIdx.x];
+        }
+    }
+}
+
+// Block (bx, by) -> (n tile, m tile).  For a lower-triangular B the cost of
+// an n tile grows with its index; launching the expensive tiles first
+// shortens the tail.
+__device__ inline void tile_coords(int tri, int ntn, int &tn, int &tm)
+{
+    tn = blockIdx.x;
+    tm = blockIdx.y;
+    if (tri) tn = ntn - 1 - tn;
+}
+
+// ---------------------------------------------------------------- FP32
+constexpr int BK32 = 16;
+constexpr int LDS32 = BK32 + 4;  // row pitch (floats): conflict-free 16-byte reads
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, size_t lda,
+                                                   const float *__restrict__ B, size_t ldb, int Kp,
+                                                   int tri, GemmEpi ep)
+{
+    __shared__ __attribute__((aligned(16))) float As[2][BM * LDS32];
+    __shared__ __attribute__((aligned(16))) float Bs[2][BN * LDS32];
+    __shared__ float red[2 * BM];
+
+    int tn, tm;
+    tile_coords(tri, gridDim.x, tn, tm);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int kend = tri ? min(Kp, n0 + BN) : Kp;
+    const int nk = kend / BK32;
+
+    // global -> register staging: thread t moves rows t/4 and t/4 + 64,
+    // columns 4 (t % 4) .. +3 of each operand tile
+    const int lr = tid >> 2, lc = (tid & 3) * 4;
+    const float *Ag0 = A + (size_t)(m0 + lr) * lda + lc;
+    const float *Ag1 = A + (size_t)(m0 + lr + 64) * lda + lc;
+    const float *Bg0 = B + (size_t)(n0 + lr) * ldb + lc;
+    const float *Bg1 = B + (size_t)(n0 + lr + 64) * ldb + lc;
+
+    floatx4 ra0, ra1, rb0, rb1;
+    auto gload = [&](int kt) {
+        const int k = kt * BK32;
+        ra0 = *(const floatx4 *)(Ag0 + k);
+        ra1 = *(const floatx4 *)(Ag1 + k);
+        rb0 = *(const floatx4 *)(Bg0 + k);
+        rb1 = *(const floatx4 *)(Bg1 + k);
+    };
+    auto sstore = [&](int buf) {
+        *(floatx4 *)(&As[buf][lr * LDS32 + lc]) = ra0;
+        *(floatx4 *)(&As[buf][(lr + 64) * LDS32 + lc]) = ra1;
+        *(floatx4 *)(&Bs[buf][lr * LDS32 + lc]) = rb0;
+        *(floatx4 *)(&Bs[buf][(lr + 64) * LDS32 + lc]) = rb1;
+    };
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.0f;
+
+    const int half = lane >> 5, l32 = lane & 31;
+    if (nk > 0) {
+        gload(0);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+        // fragments: lane (half, l32) holds k = 8 half .. 8 half + 7 of row l32
+        floatx4 a[2][2], b[2][2];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) {
+            const float *p = &As[cur][(wm * 64 + mi * 32 + l32) * LDS32 + half * 8];
+            a[mi][0] = *(const floatx4 *)p;
+            a[mi][1] = *(const floatx4 *)(p + 4);
+        }
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+            const float *p = &Bs[cur][(wn * 64 + ni * 32 + l32) * LDS32 + half * 8];
+            b[ni][0] = *(const floatx4 *)p;
+            b[ni][1] = *(const floatx4 *)(p + 4);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                        a[mi][kk >> 2][kk & 3], b[ni][kk >> 2][kk & 3], acc[mi][ni], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    gemm_epilogue<EPI>(acc, ep, m0, n0, tn, wm, wn, red);
+}
+
+// ---------------------------------------------------------------- split BF16
+constexpr int BK16 = 32;
+constexpr int LDS16 = BK16 + 8;  // row pitch (bf16): 80 bytes
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__ Ahi, const __bf16 *__restrict__ Alo,
+                                                      size_t lda, const __bf16 *__restrict__ Bhi,
+                                                      const __bf16 *__restrict__ Blo, size_t ldb, int Kp,
+                                                      int tri, int plain, GemmEpi ep)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 Ash[2][BM * LDS16];
+    __shared__ __attribute__((aligned(16))) __bf16 Asl[2][BM * LDS16];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsh[2][BN * LDS16];
+    __shared__ __attribute__((aligned(16))) __bf16 Bsl[2][BN * LDS16];
+    __shared__ float red[2 * BM];
+
+    int tn, tm;
+    tile_coords(tri, gridDim.x, tn, tm);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int kend = tri ? min(Kp, n0 + BN) : Kp;
+    const int nk = kend / BK16;
+
+    // tile = 128 rows x 32 bf16 = 128 x 64 bytes: thread t moves 16 bytes at
+    // rows t/4 and t/4 + 64, columns 8 (t % 4) .. +7
+    const int lr = tid >> 2, lc = (tid & 3) * 8;
+    const size_t a0 = (size_t)(m0 + lr) * lda + lc, a1 = (size_t)(m0 + lr + 64) * lda + lc;
+    const size_t b0 = (size_t)(n0 + lr) * ldb + lc, b1 = (size_t)(n0 + lr + 64) * ldb + lc;
+
+    bf16x8 rah0, rah1, ral0, ral1, rbh0, rbh1, rbl0, rbl1;
+    auto gload = [&](int kt) {
+        const int k = kt * BK16;
+        rah0 = *(const bf16x8 *)(Ahi + a0 + k);
+        rah1 = *(const bf16x8 *)(Ahi + a1 + k);
+        rbh0 = *(const bf16x8 *)(Bhi + b0 + k);
+        rbh1 = *(const bf16x8 *)(Bhi + b1 + k);
+        if (!plain) {
+            ral0 = *(const bf16x8 *)(Alo + a0 + k);
+            ral1 = *(const bf16x8 *)(Alo + a1 + k);
+            rbl0 = *(const bf16x8 *)(Blo + b0 + k);
+            rbl1 = *(const bf16x8 *)(Blo + b1 + k);
+        }
+    };
+    auto sstore = [&](int buf) {
+        *(bf16x8 *)(&Ash[buf][lr * LDS16 + lc]) = rah0;
+        *(bf16x8 *)(&Ash[buf][(lr + 64) * LDS16 + lc]) = rah1;
+        *(bf16x8 *)(&Bsh[buf][lr * LDS16 + lc]) = rbh0;
+        *(bf16x8 *)(&Bsh[buf][(lr + 64) * LDS16 + lc]) = rbh1;
+        if (!plain) {
+            *(bf16x8 *)(&Asl[buf][lr * LDS16 + lc]) = ral0;
+            *(bf16x8 *)(&Asl[buf][(lr + 64) * LDS16 + lc]) = ral1;
+            *(bf16x8 *)(&Bsl[buf][lr * LDS16 + lc]) = rbl0;
+            *(bf16x8 *)(&Bsl[buf][(lr + 64) * LDS16 + lc]) = rbl1;
+        }
+    };
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.0f;
+
+    const int half = lane >> 5, l32 = lane & 31;
+    if (nk > 0) {
+        gload(0);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+        // lane (half, l32) takes k = 16 half .. 16 half + 15 of row l32 as two
+        // groups of 8 (one per MFMA)
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            bf16x8 ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) {
+                const int off = (wm * 64 + mi * 32 + l32) * LDS16 + half * 16 + kk * 8;
+                ah[mi] = *(const bf16x8 *)(&Ash[cur][off]);
+                if (!plain) al[mi] = *(const bf16x8 *)(&Asl[cur][off]);
+            }
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int off = (wn * 64 + ni * 32 + l32) * LDS16 + half * 16 + kk * 8;
+                bh[ni] = *(const bf16x8 *)(&Bsh[cur][off]);
+                if (!plain) bl[ni] = *(const bf16x8 *)(&Bsl[cur][off]);
+            }
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) {
+                    if (!plain) {
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+                    }
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                }
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    gemm_epilogue<EPI>(acc, ep, m0, n0, tn, wm, wn, red);
+}
+
+// ---------------------------------------------------------------- reductions
+// llk[c] = normalization - sum_t partial[c, t] / 2, summed in FP64 in a fixed
+// order (deterministic).
+template <typename T>
+__global__ void llk_reduce_kernel(const T *__restrict__ partial, int ntiles, int C, double normalization,
+                                  double *__restrict__ llk)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const T *p = partial + (size_t)c * ntiles;
+    double s = 0.0;
+    for (int t = 0; t < ntiles; ++t) s += (double)p[t];
+    llk[c] = normalization - 0.5 * s;
+}
+
+// FP64 reference-precision whitening for small problems: one block per chain,
+// |W r|^2 with W lower triangular.
+__global__ void whiten_f64_kernel(const double *__restrict__ R, size_t ldr, const double *__restrict__ W,
+                                  size_t ldw, int N, int C, double normalization, double *__restrict__ llk)
+{
+    const int c = blockIdx.x;
+    const double *r = R + (size_t)c * ldr;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double *w = W + (size_t)i * ldw;
+        double v = 0.0;
+        for (int j = 0; j <= i; ++j) v += w[j] * r[j];
+        acc += v * v;
+    }
+    __shared__ double red[256 / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) s += red[w];
+        llk[c] = normalization - 0.5 * s;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_gemm_f32(const float *A, size_t lda, const float *B, size_t ldb, int Mp, int Np, int Kp,
+                           int tri, int epi, const GemmEpi &ep, hipStream_t st)
+{
+    dim3 grid(Np / BN, Mp / BM), block(256);
+    if (epi == EPI_STORE)
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+    else if (epi == EPI_SUMSQ)
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+    else
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ_BIAS>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, const void *Bhi, const void *Blo,
+                              size_t ldb, int Mp, int Np, int Kp, int tri, int plain, int epi,
+                              const GemmEpi &ep, hipStream_t st)
+{
+    dim3 grid(Np / BN, Mp / BM), block(256);
+    const __bf16 *ah = (const __bf16 *)Ahi, *al = (const __bf16 *)Alo;
+    const __bf16 *bh = (const __bf16 *)Bhi, *bl = (const __bf16 *)Blo;
+    if (epi == EPI_STORE)
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+    else if (epi == EPI_SUMSQ)
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+    else
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ_BIAS>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+    return hipGetLastError();
+}
+
+hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
+                             double *llk, hipStream_t st)
+{
+    dim3 block(256), grid((C + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL((llk_reduce_kernel<double>), grid, block, 0, st, (const double *)partial, ntiles, C,
+                           normalization, llk);
+    else
+        hipLaunchKernelGGL((llk_reduce_kernel<float>), grid, block, 0, st, (const float *)partial, ntiles, C,
+                           normalization, llk);
+    return hipGetLastError();
+}
+
+hipError_t launch_whiten_f64(const double *R, size_t ldr, const double *W, size_t ldw, int N, int C,
+                             double normalization, double *llk, hipStream_t st)
+{
+    hipLaunchKernelGGL(whiten_f64_kernel, dim3(C), dim3(256), 0, st, R, ldr, W, ldw, N, C, normalization, llk);
+    return hipGetLastError();
+}
+
+}  // namespace altar

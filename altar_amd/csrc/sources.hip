@@ -1,0 +1,298 @@
+// Forward models (Mogi point source, compound dislocation model) fused with
+// the residual r = LOS.u - shift - d and one of three epilogues:
+//   RES_F32   write r (FP32) into a padded C x ldr matrix, the A operand of
+//             the whitening GEMM;
+//   RES_BF16  write r as a BF16 hi/lo pair (r = hi + lo to ~2^-16) for the
+//             split-precision whitening GEMM;
+//   DIAG      whiten by a diagonal (1/sigma_i), square and reduce: one
+//             partial sum per (chain, observation tile).
+//
+// Launch shape: one thread per (chain, observation).  A block is 256
+// threads = 4 wavefronts over 256 consecutive observations of ONE chain, so
+// every per-chain constant is wave-uniform and read with scalar loads.  The
+// grid is (C, ceil(N / 256)).
+#include "common.hpp"
+#include "kernels.hpp"
+#include "cdm_math.hpp"
+
+#include <hip/hip_bf16.h>
+
+namespace altar {
+
+constexpr int OBS_TILE = 256;
+
+// ------------------------------------------------------------------ epilogue
+template <int MODE, typename T>
+struct Epilogue;
+
+template <typename T>
+__device__ inline void block_partial(T v, T *partial, int chain, int tile, int ntiles)
+{
+    __shared__ T red[OBS_TILE / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T s = red[0] + red[1] + red[2] + red[3];
+        partial[(size_t)chain * ntiles + tile] = s;
+    }
+}
+
+// ------------------------------------------------------------------ Mogi
+// models/mogi/lib/libmogi/Source.cc:65-118: u = (x nx + y ny - d nz) (nu-1)
+// 10^s / (pi R^3) - shift with x = x_src - x_obs; residual u - d_obs.
+// Per-chain constants (prepared in FP64): x_src, y_src, d_src, (nu-1)10^s/pi,
+// and the dataset shifts.
+__global__ void mogi_prepare_kernel(const double *__restrict__ theta, int C, int P, int xIdx,
+                                    int dIdx, int sIdx, int offsetIdx, int nsets, double nu,
+                                    double *__restrict__ prm /* C x MOGI_PRM */)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double *t = theta + (size_t)c * P;
+    double *o = prm + (size_t)c * MOGI_PRM;
+    o[0] = t[xIdx];
+    o[1] = t[xIdx + 1];
+    o[2] = t[dIdx];
+    o[3] = (nu - 1.0) * pow(10.0, t[sIdx]) / PI;
+    for (int k = 0; k < MAX_SETS; ++k) o[4 + k] = (offsetIdx >= 0 && k < nsets) ? t[offsetIdx + k] : 0.0;
+}
+
+template <typename T>
+__device__ inline T select_shift(const double *__restrict__ o, int nsets, int id)
+{
+    T shift = T(0);
+    for (int k = 0; k < nsets; ++k) shift = (id == k) ? (T)o[4 + k] : shift;
+    return shift;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(OBS_TILE) void mogi_kernel(ObsView obs, const double *__restrict__ prm,
+                                                        int nsets, int C, Out out)
+{
+    const int chain = blockIdx.x;
+    const int n = blockIdx.y * OBS_TILE + threadIdx.x;
+    const double *o = prm + (size_t)chain * MOGI_PRM;
+    T val = T(0);
+    if (n < obs.N) {
+        T xs = (T)o[0], ys = (T)o[1], ds = (T)o[2], cc = (T)o[3];
+        T x, y, nx, ny, nz, d;
+        if constexpr (sizeof(T) == 8) {
+            x = xs - obs.xd[n]; y = ys - obs.yd[n];
+            nx = obs.lxd[n]; ny = obs.lyd[n]; nz = obs.lzd[n]; d = obs.dd[n];
+        } else {
+            x = xs - obs.x[n]; y = ys - obs.y[n];
+            nx = obs.lx[n]; ny = obs.ly[n]; nz = obs.lz[n]; d = obs.d[n];
+        }
+        T R2 = x * x + y * y + ds * ds;
+        T R = sqrt(R2);
+        T cr3 = cc / (R2 * R);
+        T u = (x * nx + y * ny - ds * nz) * cr3;
+        u -= select_shift<T>(o, nsets, obs.oid[n]);
+        val = u - d;
+    }
+    if constexpr (MODE == MODE_DIAG) {
+        T w = n < obs.N ? (sizeof(T) == 8 ? (T)obs.wdd[n] : (T)obs.wd[n]) : T(0);
+        T z = val * w;
+        block_partial<T>(z * z, (T *)out.partial, chain, blockIdx.y, gridDim.y);
+    } else if constexpr (MODE == MODE_RES_F32) {
+        if (n < obs.N) out.r32[(size_t)chain * out.ld + n] = (float)val;
+    } else if constexpr (MODE == MODE_RES_F64) {
+        if (n < obs.N) out.r64[(size_t)chain * out.ld + n] = (double)val;
+    } else {
+        if (n < obs.N) {
+            float v = (float)val;
+            __hip_bfloat16 hi = __float2bfloat16(v);
+            __hip_bfloat16 lo = __float2bfloat16(v - __bfloat162float(hi));
+            out.rhi[(size_t)chain * out.ld + n] = hi;
+            out.rlo[(size_t)chain * out.ld + n] = lo;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ CDM
+// Geometry preparation, one thread per chain, FP64 (libcdm.py:163-273).
+__global__ void cdm_prepare_kernel(const double *__restrict__ theta, int C, int P, CdmLayout L,
+                                   double nu, double fp64_sin, cdm::Geom<float> *__restrict__ g32,
+                                   cdm::Geom<double> *__restrict__ g64, unsigned *__restrict__ flags,
+                                   float *__restrict__ shifts /* C x MAX_SETS */)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double *t = theta + (size_t)c * P;
+    cdm::Geom<double> G;
+    unsigned f = cdm::make_geometry(t[L.xIdx], t[L.xIdx + 1], t[L.dIdx], t[L.aXIdx],
+                                        t[L.aXIdx + 1], t[L.aXIdx + 2], t[L.omegaXIdx],
+                                        t[L.omegaXIdx + 1], t[L.omegaXIdx + 2], t[L.openingIdx],
+                                        nu, fp64_sin, G);
+    g64[c] = G;
+    cdm::Geom<float> Gf;
+    cdm::convert(G, Gf);
+    g32[c] = Gf;
+    flags[c] = f;
+    for (int k = 0; k < MAX_SETS; ++k)
+        shifts[(size_t)c * MAX_SETS + k] =
+            (L.offsetIdx >= 0 && k < L.nsets) ? (float)t[L.offsetIdx + k] : 0.0f;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::Geom<float> *__restrict__ g32,
+                                                       const cdm::Geom<double> *__restrict__ g64,
+                                                       const unsigned *__restrict__ flags,
+                                                       const float *__restrict__ shifts, int nsets,
+                                                       float nu, Out out)
+{
+    const int chain = blockIdx.x;
+    const int n = blockIdx.y * OBS_TILE + threadIdx.x;
+    const cdm::Geom<float> &G = g32[chain];
+    const unsigned fl = flags[chain];
+    float val = 0.0f;
+    if (n < obs.N) {
+        if (fl & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) {
+            val = __builtin_nanf("");
+        } else {
+            const float X = obs.x[n], Y = obs.y[n];
+            float dx[12], dy[12], r[12], ir[12], rpa[12], irpa[12];
+#pragma unroll
+            for (int v = 0; v < 12; ++v) {
+                dx[v] = X - G.vx[v];
+                dy[v] = Y - G.vy[v];
+                float a = G.va[v];
+                r[v] = sqrtf(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
+                ir[v] = 1.0f / r[v];
+                rpa[v] = r[v] + a;
+                irpa[v] = 1.0f / rpa[v];
+            }
+            const float k = 1.0f - 2.0f * nu;
+            float ue = 0.0f, un = 0.0f, uv = 0.0f;
+            double due = 0.0, dun = 0.0, duv = 0.0;
+#pragma unroll
+            for (int s = 0; s < 12; ++s) {
+                if (fl & cdm::skip_bit(s)) continue;
+                const int a = cdm::side_vA(s), b = cdm::side_vB(s);
+                if (fl & cdm::fp64_bit(s)) {
+                    // near-vertical side: the pair is evaluated in FP64
+                    const cdm::Geom<double> &H = g64[chain];
+                    double Xd = obs.xd[n], Yd = obs.yd[n];
+                    double dxa = Xd - H.vx[a], dya = Yd - H.vy[a], aa = H.va[a];
+                    double dxb = Xd - H.vx[b], dyb = Yd - H.vy[b], ab = H.va[b];
+                    double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
+                    double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
+                    double e, nn, u;
+                    cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya,
+                                           aa, ra, 1.0 / ra, ra + aa, 1.0 / (ra + aa), ab, rb,
+                                           1.0 / rb, rb + ab, 1.0 / (rb + ab), e, nn, u);
+                    due += e; dun += nn; duv += u;
+                } else {
+                    float e, nn, u;
+                    cdm::side_disp<float>(G.side[s], nu, k, dx[a], dy[a], G.va[a], r[a], ir[a],
+                                          rpa[a], irpa[a], G.va[b], r[b], ir[b], rpa[b], irpa[b],
+                                          e, nn, u);
+                    ue += e; un += nn; uv += u;
+                }
+            }
+            float fe = ue + (float)due, fn = un + (float)dun, fv = uv + (float)duv;
+            float u = fe * obs.lx[n] + fn * obs.ly[n] + fv * obs.lz[n];
+            const float *sh = shifts + (size_t)chain * MAX_SETS;
+            float shift = 0.0f;
+            const int id = obs.oid[n];
+            for (int q = 0; q < nsets; ++q) shift = (id == q) ? sh[q] : shift;
+            val = u - shift - obs.d[n];
+        }
+    }
+    if constexpr (MODE == MODE_DIAG) {
+        float w = n < obs.N ? obs.wd[n] : 0.0f;
+        float z = val * w;
+        block_partial<float>(z * z, (float *)out.partial, chain, blockIdx.y, gridDim.y);
+    } else if constexpr (MODE == MODE_RES_F32) {
+        if (n < obs.N) out.r32[(size_t)chain * out.ld + n] = val;
+    } else if constexpr (MODE == MODE_RES_F64) {
+        if (n < obs.N) out.r64[(size_t)chain * out.ld + n] = (double)val;
+    } else {
+        if (n < obs.N) {
+            __hip_bfloat16 hi = __float2bfloat16(val);
+            __hip_bfloat16 lo = __float2bfloat16(val - __bfloat162float(hi));
+            out.rhi[(size_t)chain * out.ld + n] = hi;
+            out.rlo[(size_t)chain * out.ld + n] = lo;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+hipError_t launch_mogi(const MogiArgs &a, int mode, int fp64, hipStream_t st)
+{
+    dim3 pb(256), pg((a.C + 255) / 256);
+    hipLaunchKernelGGL(mogi_prepare_kernel, pg, pb, 0, st, a.theta, a.C, a.P, a.xIdx, a.dIdx,
+                       a.sIdx, a.offsetIdx, a.nsets, a.nu, a.prm);
+    dim3 grid(a.C, (a.obs.N + OBS_TILE - 1) / OBS_TILE), block(OBS_TILE);
+#define MOGI_LAUNCH(T, M) hipLaunchKernelGGL((mogi_kernel<T, M>), grid, block, 0, st, a.obs, a.prm, a.nsets, a.C, a.out)
+    if (fp64) {
+        if (mode == MODE_DIAG) MOGI_LAUNCH(double, MODE_DIAG);
+        else if (mode == MODE_RES_F64) MOGI_LAUNCH(double, MODE_RES_F64);
+        else if (mode == MODE_RES_F32) MOGI_LAUNCH(double, MODE_RES_F32);
+        else MOGI_LAUNCH(double, MODE_RES_BF16);
+    } else {
+        if (mode == MODE_DIAG) MOGI_LAUNCH(float, MODE_DIAG);
+        else if (mode == MODE_RES_F64) MOGI_LAUNCH(float, MODE_RES_F64);
+        else if (mode == MODE_RES_F32) MOGI_LAUNCH(float, MODE_RES_F32);
+        else MOGI_LAUNCH(float, MODE_RES_BF16);
+    }
+#undef MOGI_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_cdm_prepare(const CdmArgs &a, hipStream_t st)
+{
+    dim3 pb(64), pg((a.C + 63) / 64);
+    hipLaunchKernelGGL(cdm_prepare_kernel, pg, pb, 0, st, a.theta, a.C, a.P, a.L, a.nu, a.fp64_sin,
+                       a.g32, a.g64, a.flags, a.shifts);
+    return hipGetLastError();
+}
+
+hipError_t launch_cdm(const CdmArgs &a, int mode, hipStream_t st)
+{
+    hipError_t e = launch_cdm_prepare(a, st);
+    if (e != hipSuccess) return e;
+    return launch_cdm_main(a, mode, st);
+}
+
+hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
+{
+    dim3 grid(a.C, (a.obs.N + OBS_TILE - 1) / OBS_TILE), block(OBS_TILE);
+    float nu = (float)a.nu;
+#define CDM_LAUNCH(M) hipLaunchKernelGGL((cdm_kernel<M>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, a.out)
+    if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG);
+    else if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32);
+    else if (mode == MODE_RES_F64) CDM_LAUNCH(MODE_RES_F64);
+    else CDM_LAUNCH(MODE_RES_BF16);
+#undef CDM_LAUNCH
+    return hipGetLastError();
+}
+
+}  // namespace altar
+
+namespace altar {
+
+// Model-side verify of the CDM: a source whose rectangles reach above the
+// free surface is rejected (the reference raises ValueError there,
+// models/cdm/cdm/libcdm.py:219-224).  mask[c] += 1.
+__global__ void cdm_verify_kernel(const double *__restrict__ theta, int C, int P, CdmLayout L,
+                                  int *__restrict__ mask)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double *t = theta + (size_t)c * P;
+    double V[12][3];
+    bool above = cdm::make_vertices(t[L.xIdx], t[L.xIdx + 1], t[L.dIdx], t[L.aXIdx], t[L.aXIdx + 1],
+                                    t[L.aXIdx + 2], t[L.omegaXIdx], t[L.omegaXIdx + 1], t[L.omegaXIdx + 2], V);
+    if (above) mask[c] += 1;
+}
+
+hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int *mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(cdm_verify_kernel, dim3((C + 63) / 64), dim3(64), 0, st, theta, C, P, L, mask);
+    return hipGetLastError();
+}
+
+}  // namespace altar

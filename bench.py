@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the CATMIP inner loop on one MI355X.
+
+Workload: CDM inversion, 5,000 LOS points on a 100 x 50 lattice over 30 x 30
+km, 16,384 chains, 10 parameters, full exponential data covariance (sill
+64 mm^2, range 5 km), FP32 forward with FP64 near-singular sides, FP32 MFMA
+whitening.  A step is one Metropolis step of every chain at a fixed
+temperature: Philox proposal, support and free-surface verify, prior, CDM
+forward + residual, whitening GEMM + reduction, accept/reject.  The sample
+state is resident on the GPU when the timed region starts.
+
+Prints one JSON line: likelihood evaluations per second (value), accepted
+samples per second, the roofline of the dominant kernel (HIP-event durations
+on its launch stream inside the timed region) and the FP64 CPU oracle timed
+on a bounded sample of chains on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+# MI355X published peaks
+PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic FP32 flops per (chain, observation) of the CDM forward + residual,
+# counted from altar_amd/csrc/cdm_math.hpp and sources.hip (fused multiply-add
+# = 2, divide/sqrt/log/atan2 = 1 each); see the census in DESIGN.md.
+CDM_FLOPS_PER_PAIR = 3252
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--chains", type=int, default=16384)
+    ap.add_argument("--obs", type=int, default=5000)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(case, theta, seconds):
+    """FP64 oracle (oracle/oracle.c) on one core: CDM residuals + whitened L2."""
+    from oracle import oracle as o
+    from altar_amd import norms
+    layout = (0, 2, 3, 4, 7, -1)
+    W = o.exact_whitener(case["cd"])
+    norm = norms.normalization(case["cd"])
+    args = (case["x"], case["y"], case["los"], case["d"], case["oid"], 0.25)
+
+    def run(n):
+        t0 = time.perf_counter()
+        r = o.cdm_residuals(theta[:n], layout, *args)
+        o.l2_llk(r, W, norm)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    n = int(max(1, min(len(theta), seconds / max(t1, 1e-6))))
+    t = run(n)
+    return {"value": n / t, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} chains x {len(case['x'])} obs, FP64 CDM + full-Cd whitening, 1 thread, {t:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(chain-parallel sharding is described in DESIGN.md, not built)"}))
+        return 2
+    import torch
+    from altar_amd import Job, _lib
+    from altar_amd.bayesian import Annealer, COV, Metropolis
+    from altar_amd.cases import cdm_case
+    from altar_amd.cdm import CDM
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    C, N = args.chains, args.obs
+    t_setup = time.perf_counter()
+    case = cdm_case(N=N)
+    job = Job(chains=C, steps=1, seed=777 + 2)
+    model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
+                nu=0.25, precision=args.precision, job=job)
+    model.initialize()
+    annealer = Annealer(Metropolis(), COV(), seed=job.seed).initialize(job)
+    annealer.model = model
+    worker = annealer.worker
+    worker.start(annealer)
+    worker.cool(annealer)           # first beta stage: beta, Sigma, resampling
+    step = worker.step
+    sampler = annealer.sampler
+    sampler.prepareSamplingPDF(annealer, step)
+    sampler.steps = args.warmup
+    sampler.walkChains(annealer, step)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+
+    src = model.source
+    _lib.call("altar_source_timing", src.handle, 1)
+    torch.cuda.synchronize()
+    sampler.steps = args.steps
+    t0 = time.perf_counter()
+    acc, rej, unl = sampler.walkChains(annealer, step)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("altar_source_timing", src.handle, 0)
+    import ctypes
+    ms = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_longlong * 4)()
+    _lib.call("altar_source_timers", src.handle, ms, cnt, 4)
+
+    evals = C * args.steps
+    value = evals / elapsed
+    fwd_ms = ms[0] / max(cnt[0], 1)
+    gemm_ms = ms[1] / max(cnt[1], 1)
+    Np = ((N + 127) // 128) * 128
+    whiten_flops = C * N * (N + 1)          # lower-triangular |W r|^2: N(N+1) per chain
+    fwd_flops = C * N * CDM_FLOPS_PER_PAIR
+    kernels = {
+        "cdm_forward": {"avg_ms": fwd_ms, "tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12 if fwd_ms else None,
+                        "bound": "valu", "peak": PEAK_FP32_TFLOPS},
+        "whitening_gemm": {"avg_ms": gemm_ms,
+                           "tflops": whiten_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms else None,
+                           "bound": "mfma",
+                           "peak": PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_BF16_TFLOPS},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    kd = kernels[dom]
+    # whitening: bytes the residual matrix and W occupy per launch (lower bound of HBM traffic)
+    traffic = None
+    roofline = {"bound": kd["bound"], "achieved": kd["tflops"], "peak": kd["peak"], "unit": "TFLOP/s",
+                "frac": kd["tflops"] / kd["peak"] if kd["tflops"] else None, "traffic": traffic,
+                "kernel": dom}
+    out = {
+        "metric": "likelihood evals/s and accepted samples/s, CDM 5k obs: plain vs surrogate-DA",
+        "value": value,
+        "unit": "likelihood evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision if args.precision == "fp32" else args.precision + "+fp32acc",
+        "data": "synthetic (CDM lattice case, exponential Cd, seeded)",
+        "config": {"workload": "CDM plain Metropolis step, 16,384 chains x 5,000 LOS, full Cd",
+                   "chains": C, "observations": N, "parameters": model.parameters,
+                   "beta": step.beta, "precision": args.precision, "parallelism": "single GPU"},
+        "accepted_samples_per_s": acc / elapsed,
+        "acceptance": {"accepted": acc, "rejected": rej, "unlikely": unl},
+        "kernels": kernels,
+        "roofline": roofline,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        theta = step.theta.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(case, theta, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

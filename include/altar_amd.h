@@ -1,0 +1,198 @@
+/*
+ * altar_amd — C ABI of the MI355X (gfx950) CATMIP inner loop.
+ *
+ * Plain pointers and sizes only.  "dev" pointers are device memory on the
+ * current HIP device; "host" pointers are host memory.  `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Every entry point
+ * returns 0 on success and a nonzero ALTAR_E* code on failure, with a
+ * message available from altar_last_error() (thread local).  No C++
+ * exception crosses this boundary.  Handles are not thread safe: one host
+ * thread per handle.  Calls are asynchronous on `stream` unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   models/mogi/ext/mogi/source.cc:44-571       libmogi newSource/data/locations/los/oid/layout/
+ *                                                displacements/residuals (CPython capsules)
+ *   models/mogi/ext/cudamogi/source.cc:54-488   libcudamogi (same verbs, fused residuals)
+ *   models/cdm/ext/cdm/source.cc:44-571         libcdm (same verbs, CDM layout)
+ *   models/cdm/ext/cudacdm/source.cc            libcudacdm
+ *   models/linear/linear/Linear.py:151-185      Linear.dataLikelihood (dgemm + L2 loop)
+ *   altar/altar/norms/L2.py:39-52               L2.withCovariance (dtrmv + dnrm2), per sample
+ *   altar/altar/bayesian/Metropolis.py:93-281   prepareSamplingPDF/displace/walkChains accept loop
+ *   altar/altar/distributions/Uniform.py:44-72  Uniform.verify
+ *   altar/altar/distributions/Base.py:47-79     initializeSample / priorLikelihood
+ *   altar/ext/dbeta.cc:39-199                   libaltar cov / dbeta_grid (COV.cc:222-368)
+ *   altar/altar/bayesian/COV.py:105-251         computeCovariance / rank
+ */
+#ifndef ALTAR_AMD_H
+#define ALTAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ALTAR_OK 0
+#define ALTAR_EINVAL 1   /* bad argument (the reference raises TypeError, ext/mogi/source.cc:95-107) */
+#define ALTAR_EHIP 2     /* HIP runtime failure */
+#define ALTAR_ENOMEM 3   /* allocation failure */
+#define ALTAR_ESTATE 4   /* call out of order (e.g. llk before observations) */
+
+/* forward models */
+#define ALTAR_MOGI 0
+#define ALTAR_CDM 1
+#define ALTAR_LINEAR 2
+
+/* whitening operators (|W r|^2 with W lower triangular) */
+#define ALTAR_WHITEN_DIAG 1
+#define ALTAR_WHITEN_LOWER 2
+
+/* arithmetic of the forward model + whitening */
+#define ALTAR_FP64 0          /* FP64 everywhere (small problems, error studies) */
+#define ALTAR_FP32 1          /* FP32 forward, FP32 MFMA whitening */
+#define ALTAR_BF16 2          /* FP32 forward, BF16 MFMA whitening (FP32 accumulate) */
+#define ALTAR_BF16_REFINED 3  /* as ALTAR_BF16 plus one split-precision correction step */
+
+typedef struct altar_source altar_source;
+typedef struct altar_prior altar_prior;
+
+const char *altar_last_error(void);
+int altar_version(void);
+
+/* ---- sources: one handle per forward model (replaces newSource, ext/mogi/source.cc:44-63) */
+
+/* model = ALTAR_MOGI | ALTAR_CDM | ALTAR_LINEAR; P parameters, N observations,
+ * nu the Poisson ratio (ignored for ALTAR_LINEAR) */
+int altar_source_create(altar_source **out, int model, int P, int N, double nu);
+int altar_source_destroy(altar_source *h);
+
+/* Parameter layout (replaces layout(), ext/mogi/source.cc:373 "O!kkkk" and
+ * ext/cdm/source.cc:373 "O!kkkkkk"):
+ *   Mogi: idx = {xIdx, dIdx, sIdx, offsetIdx, nsets}           (y = xIdx + 1; s = log10 dV)
+ *   CDM:  idx = {xIdx, dIdx, openingIdx, aXIdx, omegaXIdx, offsetIdx, nsets}
+ * offsetIdx = -1 with nsets = 0 means no per-dataset shifts; nsets <= 8. */
+int altar_source_layout(altar_source *h, const int *idx, int nidx);
+
+/* Observations, host FP64 (replaces locations/los/oid/data):
+ * x, y: N; los: N x 3 unit vectors; d: N observed displacements; oid: N
+ * dataset ids in [0, nsets).  Copied to the device (synchronous). */
+int altar_source_observations(altar_source *h, const double *x, const double *y, const double *los,
+                              const double *d, const int *oid);
+
+/* Linear model operator, host FP64: G is N x P row-major, d is N
+ * (replaces Linear.loadInputs/initializeResiduals, Linear.py:231-343).
+ * prewhitened = 1 declares that G and d were already multiplied by the
+ * whitener W (so |G theta - d|^2 is the whitened norm and no whitening
+ * operator is set). */
+int altar_source_green(altar_source *h, const double *G, const double *d, int prewhitened);
+
+/* Whitening operator and normalization, host FP64 (replaces cd_inv and
+ * normalization, models/mogi/mogi/Mogi.py:327-356):
+ * kind = ALTAR_WHITEN_DIAG: W is N values; kind = ALTAR_WHITEN_LOWER: W is
+ * N x N row-major lower triangular (entries above the diagonal ignored).
+ * llk = normalization - |W r|^2 / 2. */
+int altar_source_whitening(altar_source *h, int kind, const double *W, double normalization);
+
+/* forwardModelBatched: predicted LOS displacements minus the dataset shift
+ * (Source::displacements, libmogi/Source.cc:65-118), FP64 out, C x ldo rows.
+ * precision ALTAR_FP64 or ALTAR_FP32 (the arithmetic used). */
+int altar_source_forward(altar_source *h, const double *theta_dev, int C, double *pred_dev, size_t ldo,
+                         int precision, void *stream);
+
+/* residuals = forward - d (Source::residuals, libmogi/Source.cc:121-154;
+ * libcudamogi fuses both, libcudamogi/Source.cc:458-474), FP64 out. */
+int altar_source_residuals(altar_source *h, const double *theta_dev, int C, double *res_dev, size_t ldr,
+                           int precision, void *stream);
+
+/* Data log-likelihood of C samples (theta_dev: C x P FP64 row-major):
+ * llk_dev[c] = normalization - |W (forward(theta_c) - d)|^2 / 2
+ * (Native/Fast strategies, models/mogi/mogi/Fast.py:60-94). */
+int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
+                          void *stream);
+
+/* Model-specific constraints: mask_dev[c] += 1 for invalid samples (CDM: a
+ * source above the free surface, where libcdm.py:219-224 raises).  No-op
+ * for Mogi and linear. */
+int altar_source_verify(altar_source *h, const double *theta_dev, int C, int *mask_dev, void *stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream (no
+ * reference counterpart; the reference's Profiler.py times wall clock).
+ * Slots: 0 forward-model kernel, 1 whitening / linear GEMM.  altar_source_timers
+ * waits for the recorded events, returns the summed milliseconds and launch
+ * counts per slot, and clears them. */
+int altar_source_timing(altar_source *h, int enable);
+int altar_source_timers(altar_source *h, double *ms, long long *count, int nslots);
+
+/* ---- priors (Contiguous psets of Uniform / Gaussian, altar/altar/models/Contiguous.py) */
+
+/* kind[j] = 0: uniform on [a[j], b[j]]; kind[j] = 1: gaussian, mean a[j], sigma b[j] */
+int altar_prior_create(altar_prior **out, int P, const int *kind, const double *a, const double *b);
+int altar_prior_destroy(altar_prior *h);
+/* initializeSample (Base.py:47-57): theta_dev[c, :] drawn from the prep
+ * distribution with the Philox stream (seed, stage, attempt); only rows
+ * with only_dev[c] != 0 when only_dev is non-NULL */
+int altar_prior_sample(altar_prior *h, double *theta_dev, int C, uint64_t seed, uint32_t stage,
+                       uint32_t attempt, const int *only_dev, void *stream);
+/* priorLikelihood (Base.py:60-79): llk_dev[c] = sum_j log pdf_j(theta[c, j]) */
+int altar_prior_llk(altar_prior *h, const double *theta_dev, int C, double *llk_dev, void *stream);
+/* verify (Uniform.py:44-72): mask_dev[c] += 1 if a parameter leaves its support */
+int altar_prior_verify(altar_prior *h, const double *theta_dev, int C, int *mask_dev, void *stream);
+
+/* ---- Metropolis (Metropolis.py:114-259) */
+
+/* candidate = theta + (L xi)^T with xi ~ N(0, 1) from the Philox stream
+ * (seed, stage, step); L_dev is P x P lower triangular (the Cholesky factor
+ * of scaling^2 Sigma).  xi_dev (C x P) receives the normals (workspace). */
+int altar_mh_propose(const double *theta_dev, int C, int P, const double *L_dev, uint64_t seed,
+                     uint32_t stage, uint32_t step, double *xi_dev, double *cand_dev, void *stream);
+/* rows with mask_dev[c] != 0 revert to theta (Metropolis.py:177-182) */
+int altar_mh_restore(double *cand_dev, const double *theta_dev, const int *mask_dev, int C, int P,
+                     void *stream);
+/* Accept/reject (Metropolis.py:189-223) with the posterior fused
+ * (cpost = cprior + beta cdata, Bayesian.py:104-115).  Uniforms from the
+ * Philox stream (seed, stage, step) unless dice_dev (C values in (0,1]) is
+ * given.  decision_dev (optional): 0 accepted, 1 rejected, 2 unlikely.
+ * counters_dev[0..2] += (accepted, rejected, unlikely).  cpost_dev optional. */
+int altar_mh_accept(double *theta_dev, double *prior_dev, double *data_dev, double *post_dev,
+                    const double *ctheta_dev, const double *cprior_dev, const double *cdata_dev,
+                    double *cpost_dev, const int *mask_dev, double beta, int C, int P, uint64_t seed,
+                    uint32_t stage, uint32_t step, const double *dice_dev, int *decision_dev,
+                    unsigned long long *counters_dev, void *stream);
+/* Metropolis.adjustCovarianceScaling (Metropolis.py:262-281) */
+double altar_mh_scaling(double acceptance_weight, double rejection_weight, unsigned long long accepted,
+                        unsigned long long rejected, unsigned long long unlikely);
+
+/* ---- COV scheduler (COV.cc:222-368, COV.py:65-251) */
+
+/* dbeta grid search on host FP64 data (libaltar dbeta_grid): given llk
+ * (C), its median and the current beta, writes the normalized weights w
+ * (C) and returns the new beta and the attained COV. */
+int altar_cov_dbeta_grid(const double *llk_host, int C, double median, double beta, double tolerance,
+                         double target, double *w_host, double *beta_out, double *cov_out);
+/* weighted mean (P) and covariance Sigma (P x P) of theta_dev with weights w_dev */
+int altar_cov_covariance(const double *theta_dev, const double *w_dev, int C, int P, double *mean_dev,
+                         double *sigma_dev, void *stream);
+/* multinomial multiplicities (COV.py:222-251): C uniforms from the Philox
+ * stream (seed, stage) binned on the edges {0, cumsum(w)} (last edge forced
+ * to 1); count_host (C).  Host, synchronous. */
+int altar_cov_multiplicities(const double *w_host, int C, uint64_t seed, uint32_t stage, int *count_host);
+/* resampled state: row c <- old row src_dev[c] (theta, prior, data, posterior) */
+int altar_resample_gather(const double *theta_old, const double *prior_old, const double *data_old,
+                          const double *post_old, const int *src_dev, int C, int P, double *theta,
+                          double *prior, double *data, double *post, void *stream);
+/* post = prior + beta data (CoolingStep.computePosterior) */
+int altar_posterior(const double *prior_dev, const double *data_dev, double beta, int C, double *post_dev,
+                    void *stream);
+
+/* Philox normals and uniforms into device memory (the same streams the
+ * kernels draw), for parity checks */
+int altar_random_normals(double *out_dev, int C, int P, uint64_t seed, uint32_t stage, uint32_t step,
+                         uint32_t tag, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALTAR_AMD_H */

@@ -1,0 +1,213 @@
+"""GPU parity of the forward models and the data log-likelihood against the
+FP64 oracle (oracle/oracle.c), on identical theta and observations.
+
+Tolerances (stated per test):
+* Mogi FP64 path: residuals and llk to 1e-12 relative.
+* Mogi FP32 path: residuals to 1e-5 of the field's max; llk to 1e-5 relative.
+* CDM (FP32, FP64 near-vertical sides): displacement error <= 2e-4 of the
+  field's max per sample (normwise); llk relative error <= 1e-4.
+* Whitening, full Cd: FP32 MFMA llk rel. error <= 1e-4; BF16 + refinement
+  <= 1e-3; plain BF16 is reported, not bounded.
+"""
+import numpy as np
+import pytest
+import torch
+
+from altar_amd import _lib, norms
+from altar_amd.cases import mogi_case, unit_los
+from altar_amd.source import Source
+
+pytestmark = pytest.mark.gpu
+
+
+def mogi_theta(rng, C, nsets=1):
+    t = np.column_stack([rng.uniform(-10e3, 10e3, C), rng.uniform(-10e3, 10e3, C),
+                         rng.uniform(500, 8000, C), rng.uniform(5, 7, C)] +
+                        [rng.uniform(-0.1, 0.1, C) for _ in range(nsets)])
+    return t
+
+
+def cdm_theta(rng, C, oracle, offsets=0):
+    rows = []
+    while len(rows) < C:
+        p = [rng.uniform(-10e3, 10e3), rng.uniform(-10e3, 10e3), rng.uniform(1e3, 6e3),
+             rng.uniform(1e-3, 1.0), rng.uniform(100, 3000), rng.uniform(100, 3000), rng.uniform(100, 3000),
+             rng.uniform(-90, 90), rng.uniform(-90, 90), rng.uniform(-90, 90)]
+        p += [rng.uniform(-0.05, 0.05) for _ in range(offsets)]
+        V, bad = oracle.cdm_vertices(p[0], p[1], p[2], p[4], p[5], p[6], p[7], p[8], p[9])
+        if not bad:
+            rows.append(p)
+    return np.array(rows)
+
+
+def make_mogi(case, P):
+    N = len(case["x"])
+    src = Source(_lib.MOGI, P, N, 0.25)
+    src.layout([0, 2, 3, 4, 1])
+    src.observations(case["x"], case["y"], case["los"], case["d"], case["oid"])
+    return src
+
+
+def test_mogi_residuals_fp64_fp32(cuda, oracle):
+    print("case: mogi residuals C=256 N=1000", flush=True)
+    case = mogi_case()
+    rng = np.random.default_rng(1)
+    theta = mogi_theta(rng, 256)
+    ref = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"],
+                                case["oid"], 0.25)
+    src = make_mogi(case, 5)
+    t = torch.from_numpy(theta).to(cuda)
+    r64 = src.residuals(t, "fp64").cpu().numpy()
+    r32 = src.residuals(t, "fp32").cpu().numpy()
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert np.max(np.abs(r64 - ref) / scale) < 1e-12
+    assert np.max(np.abs(r32 - ref) / scale) < 1e-5
+
+
+def test_mogi_llk_diag(cuda, oracle):
+    print("case: mogi llk diag C=1024 N=1000", flush=True)
+    case = mogi_case()
+    rng = np.random.default_rng(2)
+    theta = mogi_theta(rng, 1024)
+    theta[:8] = case["truth"] + rng.normal(0, [10, 10, 10, 0.001, 1e-4], (8, 5))
+    res = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"],
+                                case["oid"], 0.25)
+    W = 1.0 / np.sqrt(case["cd"])
+    norm = norms.normalization(case["cd"])
+    ref = oracle.l2_llk(res, W, norm)
+    src = make_mogi(case, 5)
+    src.whitening("diag", W, norm)
+    t = torch.from_numpy(theta).to(cuda)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        out = torch.empty(1024, dtype=torch.float64, device=cuda)
+        src.data_llk(t, out, prec)
+        got = out.cpu().numpy()
+        rel = np.abs(got - ref) / np.abs(ref)
+        print(prec, "max rel llk err", rel.max())
+        assert rel.max() < tol, prec
+
+
+@pytest.mark.parametrize("N", [300, 1000])
+def test_mogi_llk_full_cd(cuda, oracle, N):
+    print(f"case: mogi llk full Cd C=512 N={N}", flush=True)
+    case = mogi_case(N=N)
+    rng = np.random.default_rng(3)
+    xy = np.stack([case["x"], case["y"]], 1)
+    cd = norms.exponential_covariance(xy, 25e-6, 5e3)
+    case["d"] = case["d"] + np.linalg.cholesky(cd) @ rng.standard_normal(N)
+    theta = mogi_theta(rng, 512)
+    theta[:16] = case["truth"] + rng.normal(0, [50, 50, 50, 0.01, 1e-3], (16, 5))
+    res = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"],
+                                case["oid"], 0.25)
+    norm = norms.normalization(cd)
+    src = make_mogi(case, 5)
+    t = torch.from_numpy(theta).to(cuda)
+    for mode in ("exact", "reference"):
+        kind, W = norms.whitener(cd, mode)
+        ref_w = oracle.exact_whitener(cd) if mode == "exact" else oracle.covariance_inverse_cholesky(cd)
+        ref = oracle.l2_llk(res, ref_w, norm)
+        if mode == "exact":
+            # the exact whitener gives the textbook quadratic form
+            q = np.einsum("ci,ci->c", res, np.linalg.solve(cd, res.T).T)
+            assert np.allclose(ref, norm - q / 2, rtol=1e-9)
+        src.whitening(kind, W, norm)
+        for prec, tol in (("fp64", 1e-10), ("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
+            out = torch.empty(512, dtype=torch.float64, device=cuda)
+            src.data_llk(t, out, prec)
+            got = out.cpu().numpy()
+            rel = np.abs(got - ref) / np.abs(ref)
+            print(mode, prec, "max rel llk err", rel.max(), "near-truth", rel[:16].max())
+            if tol is not None:
+                assert rel.max() < tol, (mode, prec)
+
+
+def test_cdm_forward_vs_oracle(cuda, oracle):
+    print("case: cdm forward C=64 N=2000", flush=True)
+    rng = np.random.default_rng(4)
+    N = 2000
+    x = rng.uniform(-15e3, 15e3, N)
+    y = rng.uniform(-15e3, 15e3, N)
+    los = unit_los(N)
+    theta = cdm_theta(rng, 64, oracle)
+    layout = (0, 2, 3, 4, 7, -1)
+    ref = oracle.cdm_residuals(theta, layout, x, y, los, np.zeros(N), np.zeros(N, np.int32), 0.25)
+    src = Source(_lib.CDM, 10, N, 0.25)
+    src.layout([0, 2, 3, 4, 7, -1, 0])
+    src.observations(x, y, los, np.zeros(N), np.zeros(N, np.int32))
+    got = src.forward(torch.from_numpy(theta).to(cuda), "fp32").cpu().numpy()
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print("cdm normwise err: median", np.median(err), "max", err.max())
+    assert err.max() < 2e-4
+
+
+def test_cdm_golden_vectors(cuda):
+    """The reference's MATLAB points and the libcdm.py vectors (tests/golden)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cdm_vectors.npz"))
+    worst = 0.0
+    for k, p in enumerate(g["params"]):
+        X0, Y0, depth, ax, ay, az, ox, oy, oz, op, nu = p
+        X = g["X"] * g["point_scale"][k]
+        Y = g["Y"] * g["point_scale"][k]
+        M = len(X)
+        for comp, los in (("ue", [1, 0, 0]), ("un", [0, 1, 0]), ("uv", [0, 0, 1])):
+            src = Source(_lib.CDM, 10, M, nu)
+            src.layout([0, 2, 3, 4, 7, -1, 0])
+            src.observations(X, Y, np.tile(los, (M, 1)), np.zeros(M), np.zeros(M, np.int32))
+            th = torch.tensor([[X0, Y0, depth, op, ax, ay, az, ox, oy, oz]], dtype=torch.float64, device=cuda)
+            got = src.forward(th, "fp32").cpu().numpy()[0]
+            ref = g[comp][k]
+            scale = max(np.abs(g["ue"][k]).max(), np.abs(g["un"][k]).max(), np.abs(g["uv"][k]).max())
+            e = np.abs(got - ref).max() / scale
+            worst = max(worst, e)
+            assert e < 2e-4, (k, comp, e)
+    print("golden worst normwise", worst)
+
+
+def test_cdm_verify_matches_vertices(cuda, oracle):
+    rng = np.random.default_rng(5)
+    C = 2000
+    theta = np.column_stack([rng.uniform(-1e4, 1e4, C), rng.uniform(-1e4, 1e4, C), rng.uniform(1e3, 6e3, C),
+                             rng.uniform(1e-3, 1, C), rng.uniform(100, 3000, (C, 3)),
+                             rng.uniform(-90, 90, (C, 3))])
+    want = np.array([oracle.cdm_vertices(t[0], t[1], t[2], t[4], t[5], t[6], t[7], t[8], t[9])[1]
+                     for t in theta], dtype=np.int32)
+    src = Source(_lib.CDM, 10, 4, 0.25)
+    src.layout([0, 2, 3, 4, 7, -1, 0])
+    mask = torch.zeros(C, dtype=torch.int32, device=cuda)
+    src.verify(torch.from_numpy(theta).to(cuda), mask)
+    assert np.array_equal(mask.cpu().numpy(), want)
+    assert 0 < want.sum() < C
+
+
+def test_cdm_llk_full_cd(cuda, oracle):
+    print("case: cdm llk full Cd C=128 N=1200", flush=True)
+    rng = np.random.default_rng(6)
+    N = 1200
+    x = rng.uniform(-15e3, 15e3, N)
+    y = rng.uniform(-15e3, 15e3, N)
+    los = unit_los(N)
+    truth = np.array([[1000.0, -500.0, 4000.0, 0.1, 1000.0, 1200.0, 800.0, 5.0, -8.0, 30.0]])
+    cd = norms.exponential_covariance(np.stack([x, y], 1), 64e-6, 5e3)
+    u = oracle.cdm_residuals(truth, (0, 2, 3, 4, 7, -1), x, y, los, np.zeros(N), np.zeros(N, np.int32), 0.25)[0]
+    d = u + np.linalg.cholesky(cd) @ rng.standard_normal(N)
+    theta = cdm_theta(rng, 128, oracle)
+    theta[:32] = truth + rng.normal(0, [20, 20, 20, 1e-3, 10, 10, 10, 0.2, 0.2, 0.2], (32, 10))
+    res = oracle.cdm_residuals(theta, (0, 2, 3, 4, 7, -1), x, y, los, d, np.zeros(N, np.int32), 0.25)
+    norm = norms.normalization(cd)
+    kind, W = norms.whitener(cd)
+    ref = oracle.l2_llk(res, oracle.exact_whitener(cd), norm)
+    src = Source(_lib.CDM, 10, N, 0.25)
+    src.layout([0, 2, 3, 4, 7, -1, 0])
+    src.observations(x, y, los, d, np.zeros(N, np.int32))
+    src.whitening(kind, W, norm)
+    t = torch.from_numpy(theta).to(cuda)
+    for prec, tol in (("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
+        out = torch.empty(128, dtype=torch.float64, device=cuda)
+        src.data_llk(t, out, prec)
+        got = out.cpu().numpy()
+        rel = np.abs(got - ref) / np.abs(ref)
+        absn = np.abs(got - ref)[:32]
+        print(prec, "max rel llk err", rel.max(), "near-truth abs err", absn.max())
+        if tol is not None:
+            assert rel.max() < tol, prec

@@ -1,0 +1,185 @@
+"""Pins of the FP64 oracle (oracle/oracle.c) against the reference's golden
+vectors, published known answers and closed forms.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+# models/cdm/tests/libcdm.py:27-46 (MATLAB outputs) with the parameters of :103-117
+KAT_X = np.array([-7, -6.90, -6.88, -6.86, -6.84, -6.82, 0, 0.50, 1.10, 6.50, 6.86, 7])
+KAT_Y = np.array([-5, -5, -5, -5, -5, -5, -4.92, -4.92, -4.92, -4.92, 4.94, 5])
+KAT_DE = np.array([-4.831270e-06, -4.886526e-06, -4.897588e-06, -4.908654e-06, -4.919721e-06, -4.930791e-06,
+                   -1.331881e-06, 9.160793e-08, 1.801943e-06, 5.766744e-06, 4.544161e-06, 4.439410e-06])
+KAT_DN = np.array([-2.988161e-06, -3.064582e-06, -3.080126e-06, -3.095757e-06, -3.111477e-06, -3.127285e-06,
+                   -1.312579e-05, -1.331642e-05, -1.315063e-05, -4.467011e-06, 3.648027e-06, 3.525858e-06])
+KAT_DV = np.array([1.799845e-06, 1.844885e-06, 1.854042e-06, 1.863249e-06, 1.872507e-06, 1.881815e-06,
+                   7.624726e-06, 7.706569e-06, 7.578237e-06, 2.489978e-06, 1.996341e-06, 1.909694e-06])
+KAT_PARAMS = dict(X0=0.5, Y0=-0.25, depth=2.75, ax=0.4, ay=0.45, az=0.8, omegaX=5.0, omegaY=-8.0,
+                  omegaZ=30.0, opening=1e-3, nu=0.25)
+
+
+def kat(o, **over):
+    p = dict(KAT_PARAMS, **over)
+    return o.cdm(KAT_X, KAT_Y, p["X0"], p["Y0"], p["depth"], p["ax"], p["ay"], p["az"], p["omegaX"],
+                 p["omegaY"], p["omegaZ"], p["opening"], p["nu"])
+
+
+def test_cdm_matlab_known_answers(oracle):
+    ue, un, uv = kat(oracle)
+    for got, want in ((ue, KAT_DE), (un, KAT_DN), (uv, KAT_DV)):
+        assert np.max(np.abs((got - want) / want)) < 1e-6   # the reference test's bar (libcdm.py:83-86)
+
+
+def test_cdm_reference_vectors(oracle):
+    """Vectors written by tests/golden/make_cdm_vectors.py from libcdm.py."""
+    g = np.load(os.path.join(GOLDEN, "cdm_vectors.npz"))
+    assert len(g["params"]) >= 40
+    for k, p in enumerate(g["params"]):
+        X = g["X"] * g["point_scale"][k]
+        Y = g["Y"] * g["point_scale"][k]
+        got = np.stack(oracle.cdm(X, Y, *p))
+        want = np.stack([g["ue"][k], g["un"][k], g["uv"][k]])
+        assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want)), k
+
+
+def test_cdm_planted_mistake_is_caught(oracle):
+    """A sign flip of one rotation angle fails the MATLAB vectors."""
+    ue, un, uv = kat(oracle, omegaY=8.0)
+    assert np.max(np.abs((uv - KAT_DV) / KAT_DV)) > 1e-3
+
+
+def test_cdm_invariants(oracle):
+    X = np.linspace(-8, 8, 25)
+    Y = np.linspace(-6, 6, 25)
+    z = kat(oracle, opening=0.0)
+    assert all(np.all(c == 0) for c in z)
+    a = np.stack(kat(oracle, opening=1e-3))
+    b = np.stack(kat(oracle, opening=3e-3))
+    assert np.allclose(b, 3 * a, rtol=1e-9, atol=0)
+    # far field -> a point source of potency dV = (ax ay + ax az + ay az) opening (full axes)
+    R = 200.0
+    ang = np.linspace(0, 2 * np.pi, 16, endpoint=False)
+    Xf, Yf = R * np.cos(ang), R * np.sin(ang)
+    ax, ay, az, op, depth, nu = 0.3, 0.4, 0.5, 1e-3, 5.0, 0.25
+    ue, un, uv = oracle.cdm(Xf, Yf, 0.0, 0.0, depth, ax, ay, az, 0.0, 0.0, 0.0, op, nu)
+    dV = (4 * ax * ay + 4 * ax * az + 4 * ay * az) * op
+    r3 = (Xf ** 2 + Yf ** 2 + depth ** 2) ** 1.5
+    uz_point = (1 - nu) * dV / np.pi * depth / r3
+    assert np.allclose(uv, uz_point, rtol=0.02)
+    with pytest.raises(ValueError):
+        oracle.cdm(Xf, Yf, 0.0, 0.0, 0.2, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, op, nu)
+
+
+def test_mogi_closed_form(oracle):
+    """Mogi (1958): u = (1 - nu) dV / (pi R^3) [x_o - x_s, y_o - y_s, d] . n - shift."""
+    rng = np.random.default_rng(0)
+    N = 200
+    ox, oy = rng.uniform(-1e4, 1e4, N), rng.uniform(-1e4, 1e4, N)
+    los = rng.normal(size=(N, 3))
+    los /= np.linalg.norm(los, axis=1, keepdims=True)
+    theta = np.array([[100.0, -200.0, 3000.0, 6.0, 0.01, -0.02]])
+    oid = rng.integers(0, 2, N).astype(np.int32)
+    d = rng.normal(size=N) * 1e-3
+    res = oracle.mogi_residuals(theta, (0, 2, 3, 4), ox, oy, los, d, oid, 0.25)[0]
+    dx, dy = ox - 100.0, oy + 200.0
+    R = np.sqrt(dx ** 2 + dy ** 2 + 3000.0 ** 2)
+    u = 0.75 * 1e6 / (np.pi * R ** 3) * (dx * los[:, 0] + dy * los[:, 1] + 3000.0 * los[:, 2])
+    want = u - np.where(oid == 0, 0.01, -0.02) - d
+    assert np.allclose(res, want, rtol=1e-12, atol=1e-18)
+    # vertical field at the epicentre: (1 - nu) dV / (pi d^2)
+    up = oracle.mogi_residuals(np.array([[0.0, 0.0, 2000.0, 5.0]]), (0, 2, 3, -1), np.zeros(1), np.zeros(1),
+                               np.array([[0.0, 0.0, 1.0]]), np.zeros(1), np.zeros(1, np.int32), 0.25)[0, 0]
+    assert np.isclose(up, 0.75 * 1e5 / (np.pi * 2000.0 ** 2), rtol=1e-14)
+
+
+def test_l2_exact_and_reference_whiteners(oracle):
+    rng = np.random.default_rng(1)
+    N = 40
+    xy = rng.uniform(0, 1e4, (N, 2))
+    dist = np.sqrt(((xy[:, None] - xy[None]) ** 2).sum(-1))
+    cd = 25e-6 * np.exp(-dist / 5e3)
+    r = rng.normal(size=(6, N)) * 5e-3
+    norm = oracle.normalization(cd)
+    sign, logdet = np.linalg.slogdet(cd)
+    assert np.isclose(norm, -0.5 * (N * np.log(2 * np.pi) + logdet))
+    exact = oracle.l2_llk(r, oracle.exact_whitener(cd), norm)
+    q = np.einsum("ci,ci->c", r, np.linalg.solve(cd, r.T).T)
+    assert np.allclose(exact, norm - q / 2, rtol=1e-10)
+    refw = oracle.l2_llk(r, oracle.covariance_inverse_cholesky(cd), norm)
+    assert not np.allclose(refw, exact, rtol=1e-3)   # |L r| != r^T Cd^-1 r for a full Cd
+    cdd = np.diag(np.diag(cd))
+    a = oracle.l2_llk(r, oracle.exact_whitener(cdd), norm)
+    b = oracle.l2_llk(r, oracle.covariance_inverse_cholesky(cdd), norm)
+    c = oracle.l2_llk(r, 1 / np.sqrt(np.diag(cd)), norm)
+    assert np.allclose(a, b, rtol=1e-13) and np.allclose(a, c, rtol=1e-13)
+
+
+def test_philox_known_answers(oracle):
+    """Random123 kat_vectors for philox4x32_10."""
+    assert list(oracle.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(oracle.philox([0xffffffff] * 4, [0xffffffff] * 2)) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6,
+                                                                      0x6d5451fd]
+    assert list(oracle.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0])) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+    u = oracle.uniforms(3, 1, 2, 1, 20000)
+    assert u.min() > 0 and u.max() <= 1 and abs(u.mean() - 0.5) < 0.01
+    z = oracle.normals(3, 1, 2, 0, 20000, 3)
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+
+
+def test_dbeta_grid_against_scan(oracle):
+    rng = np.random.default_rng(2)
+    llk = rng.normal(-1000, 40, 3000)
+    beta, cov, w = oracle.dbeta_grid(llk, 0.0)
+    assert 0 < beta < 1
+    assert abs(cov - 1.0) < 0.01 or cov >= 1.0
+    assert np.isclose(w.sum(), 1.0)
+    # COV grows with dbeta; a brute-force scan brackets the answer
+    med = oracle.median(llk)
+    grid = np.linspace(0, 1, 2001)
+    covs = np.array([oracle.cov_eval(g, llk, med)[1] for g in grid])
+    first = grid[np.argmax(covs >= 1.0 - 0.01)]
+    assert abs(beta - first) < 1e-3 + 1e-2 * first
+    # skip to beta = 1 when even the full step is below target
+    b1, c1, _ = oracle.dbeta_grid(np.full(100, -5.0) + rng.normal(0, 1e-3, 100), 0.5)
+    assert b1 == 1.0
+
+
+def test_weighted_covariance_and_multiplicities(oracle):
+    rng = np.random.default_rng(3)
+    theta = rng.normal(size=(500, 4))
+    w = rng.random(500)
+    w /= w.sum()
+    mean, sigma = oracle.weighted_covariance(theta, w)
+    assert np.allclose(mean, np.average(theta, axis=0, weights=w))
+    assert np.allclose(sigma, np.cov(theta.T, aweights=w, bias=True))
+    r = oracle.uniforms(4, 0, 0, 2, 500)
+    count = oracle.multiplicities(w, r)
+    assert count.sum() == 500
+    edges = np.concatenate([[0], np.cumsum(w)])
+    edges[-1] = 1.0
+    want = np.histogram(r, bins=edges)[0]
+    # numpy bins are [a, b); ours are (a, b]: identical unless a draw hits an edge
+    assert np.array_equal(count, want)
+
+
+def test_accept_bookkeeping(oracle):
+    rng = np.random.default_rng(4)
+    C, P = 1000, 3
+    th = rng.normal(size=(C, P))
+    pr, da = rng.normal(size=C), rng.normal(size=C)
+    po = pr + da
+    cth = rng.normal(size=(C, P))
+    cpr, cda = rng.normal(size=C), rng.normal(size=C)
+    mask = (rng.random(C) < 0.3).astype(np.int32)
+    dice = oracle.uniforms(1, 0, 0, 1, C)
+    th0, po0 = th.copy(), po.copy()
+    acc, dec = oracle.mh_accept(th, pr, da, po, cth, cpr, cda, cpr + cda, mask, dice)
+    assert acc == (dec == 0).sum() and ((dec == 1) == (mask == 1)).all()
+    diff = cpr + cda - po0
+    assert ((dec == 2) == ((mask == 0) & (np.log(dice) > diff))).all()
+    assert np.array_equal(th[dec == 0], cth[dec == 0]) and np.array_equal(th[dec != 0], th0[dec != 0])
+    assert np.isclose(oracle.mh_adjust_scaling(acc, (dec == 1).sum(), (dec == 2).sum()),
+                      min(1, max(0.1, (8 * acc / C + 1) / 9)))
