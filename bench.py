@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FP32 flops per (chain, observation) of the CDM forward + residual,
 # counted from altar_amd/csrc/cdm_math.hpp and sources.hip (fused multiply-add
 # = 2, divide/sqrt/log/atan2 = 1 each); see the census in DESIGN.md.
-CDM_FLOPS_PER_PAIR = 3252
+CDM_FLOPS_PER_PAIR = 2983
 
 
 def parse():

@@ -111,6 +111,19 @@ void oracle_uniforms(uint64_t seed, uint32_t stage, uint32_t step, uint32_t tag,
     }
 }
 
+/* two uniforms on (0,1] per (chain, word): words (0,1) and (2,3) of the block */
+void oracle_uniform_pairs(uint64_t seed, uint32_t stage, uint32_t step, uint32_t tag,
+                          int C, int W, double *out /* C x W x 2 */)
+{
+    for (int c = 0; c < C; ++c)
+        for (int k = 0; k < W; ++k) {
+            uint32_t w[4];
+            draw(seed, stage, step, (uint32_t)c, tag, (uint32_t)k, w);
+            out[((size_t)c * W + k) * 2] = u53(w[0], w[1]);
+            out[((size_t)c * W + k) * 2 + 1] = u53(w[2], w[3]);
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Mogi point source: models/mogi/lib/libmogi/Source.cc:65-118 and residuals
  * :121-154.  theta is C x P row-major; strength is log10(dV) (:78).        */

@@ -38,6 +38,7 @@ def _declare(lib):
         "oracle_philox4x32_10": (None, [_u32, _u32, _u32]),
         "oracle_normals": (None, [c_u64, c_u32, c_u32, c_u32, c_int, c_int, _d]),
         "oracle_uniforms": (None, [c_u64, c_u32, c_u32, c_u32, c_int, _d]),
+        "oracle_uniform_pairs": (None, [c_u64, c_u32, c_u32, c_u32, c_int, c_int, _d]),
         "oracle_mogi_residuals": (None, [_d, c_int, c_int, c_int, c_int, c_int, c_int,
                                          _d, _d, _d, _d, _i, c_int, c_double, _d]),
         "oracle_ang_dis_disp_surf": (None, [c_double] * 8 + [_d]),
@@ -111,6 +112,24 @@ def uniforms(seed, stage, step, tag, C):
     out = np.zeros(C)
     lib().oracle_uniforms(seed, stage, step, tag, C, _p(out))
     return out
+
+
+def uniform_pairs(seed, stage, step, tag, C, W):
+    out = np.zeros((C, W, 2))
+    lib().oracle_uniform_pairs(seed, stage, step, tag, C, W, _p(out))
+    return out
+
+
+def prep_sample(seed, attempt, kinds, lo, hi, C):
+    """Initial draws (uniform: a + (b - a)(1 - u1); gaussian: Box-Muller) from
+    the init stream (stage 0, step = attempt, tag 3)."""
+    kinds = np.asarray(kinds)
+    P = len(kinds)
+    u = uniform_pairs(seed, 0, attempt, 3, C, P)
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    uni = lo + (hi - lo) * (1.0 - u[:, :, 0])
+    gau = lo + hi * np.sqrt(-2.0 * np.log(u[:, :, 0])) * np.cos(2.0 * np.pi * u[:, :, 1])
+    return np.where(kinds == 0, uni, gau)
 
 
 # ---------------------------------------------------------------- forward models
