@@ -50,6 +50,7 @@ SIGNATURES = {
     "altar_source_residuals": (_i, [_vp, _vp, _i, _vp, _sz, _i, _vp]),
     "altar_source_data_llk": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "altar_source_verify": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "altar_source_data_llk_fields": (_i, [_vp, _vp, _i, _vp, _i, _vp, _sz, _vp]),
     "altar_source_timing": (_i, [_vp, _i]),
     "altar_source_timers": (_i, [_vp, _vp, _vp, _i]),
     "altar_prior_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp]),
@@ -68,6 +69,21 @@ SIGNATURES = {
     "altar_resample_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "altar_posterior": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "altar_random_normals": (_i, [_vp, _i, _i, _u64, _u32, _u32, _u32, _vp]),
+    "altar_gemm_f32": (_i, [_vp, _sz, _vp, _sz, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "altar_gather_rows_f32": (_i, [_vp, _sz, _vp, _i, _i, _vp, _sz, _vp]),
+    "altar_mlp_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _i]),
+    "altar_mlp_destroy": (_i, [_vp]),
+    "altar_mlp_input_scaling": (_i, [_vp, _vp, _vp]),
+    "altar_mlp_set_layer": (_i, [_vp, _i, _vp, _vp]),
+    "altar_mlp_get_layer": (_i, [_vp, _i, _vp, _vp, _i]),
+    "altar_mlp_forward": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
+    "altar_mlp_backward": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp, _vp]),
+    "altar_mlp_adam": (_i, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
+    "altar_surrogate_llk": (_i, [_vp, _i, _i, _i, _vp, _vp, _d, _d, _vp, _vp]),
+    "altar_da_screen": (_i, [_vp, _vp, _vp, _vp, _d, _i, _u64, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "altar_da_pack": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "altar_da_accept": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _i, _u64, _u32, _u32,
+                             _vp, _vp, _vp]),
 }
 
 

@@ -1,0 +1,154 @@
+"""Two-stage delayed-acceptance Metropolis with a learned surrogate.
+
+Christen and Fox (2005): stage 1 screens the proposal with the surrogate
+posterior pi*, stage 2 corrects with the exact posterior pi,
+
+    alpha1 = min(1, pi*(theta') / pi*(theta))
+    alpha2 = min(1, pi(theta') pi*(theta) / (pi(theta) pi*(theta'))),
+
+so the chain keeps pi as its stationary distribution whatever the surrogate's
+error; the exact forward model runs only for the chains that pass stage 1
+(compacted into a dense batch).  No reference counterpart: this extends
+altar/altar/bayesian/Metropolis.py:114-233.
+
+Schedule: while beta < ``train_beta`` the sampler runs plain Metropolis steps
+and stores the whitened residuals of every exact evaluation (the training
+fields).  When beta reaches ``train_beta`` (or the store is full) the
+surrogate is fitted; afterwards each stage validates the surrogate on fresh
+exact fields of the current chains and falls back to the plain path when the
+field RMSE exceeds ``max_rmse`` metres.
+"""
+import numpy as np
+import torch
+
+from .. import _lib
+from .cooling_step import CoolingStep
+from .metropolis import Metropolis
+
+
+class DelayedAcceptance(Metropolis):
+    def __init__(self, surrogate, train_beta=0.1, max_rmse=2e-3, fit_kwargs=None, validate_chains=1024,
+                 **kwds):
+        super().__init__(**kwds)
+        self.surrogate = surrogate
+        self.train_beta = train_beta
+        self.max_rmse = max_rmse
+        self.fit_kwargs = fit_kwargs or {}
+        self.validate_chains = validate_chains
+        self.mode_history = []
+        self.screened = 0
+        self.exact_evals = 0
+
+    # -- plain steps that also collect training fields
+    def _collecting(self, step):
+        return (not self.surrogate.trained and step.beta < self.train_beta and self.surrogate.space() > 0
+                and step.samples % 128 == 0)
+
+    def walkChains(self, annealer, step):
+        sur = self.surrogate
+        if not sur.trained and (step.beta >= self.train_beta or sur.space() < step.samples):
+            if sur.count >= 1024:
+                sur.fit(**self.fit_kwargs)
+        use_da = sur.trained and self._validate(annealer, step)
+        self.mode_history.append({"beta": step.beta, "mode": "da" if use_da else "plain",
+                                  "rmse": sur.validation_rmse})
+        if use_da:
+            return self._walk_da(annealer, step)
+        if self._collecting(step):
+            return self._walk_plain(annealer, step, collect=True)
+        stats = super().walkChains(annealer, step)
+        self.exact_evals += step.samples * self.steps
+        return stats
+
+    def _validate(self, annealer, step):
+        """Field RMSE of the surrogate on the current chains (exact fields)."""
+        sur = self.surrogate
+        n = min(self.validate_chains, step.samples) // 128 * 128
+        if n == 0:
+            return sur.validation_rmse is not None and sur.validation_rmse <= self.max_rmse
+        theta = step.theta[:n].contiguous()
+        llk = torch.empty(n, dtype=torch.float64, device=theta.device)
+        wr = torch.empty((n, sur.Npad), dtype=torch.float32, device=theta.device)
+        _lib.call("altar_source_data_llk_fields", self.model_source(annealer).handle, _lib.ptr(theta), n,
+                  _lib.ptr(llk), _lib.PRECISIONS[annealer.model.precision], _lib.ptr(wr), sur.Npad, _lib.stream())
+        rmse = sur.field_rmse(sur.predict_wr(theta), wr[:, :sur.N].double().cpu().numpy())
+        sur.validation_rmse = rmse
+        return rmse <= self.max_rmse
+
+    @staticmethod
+    def model_source(annealer):
+        return annealer.model.source
+
+    def _walk_plain(self, annealer, step, collect):
+        """Metropolis.walkChains with the candidate's exact llk stored as training data."""
+        model = annealer.model
+        strategy = model.strategy
+        sur = self.surrogate
+        original = strategy.dataLikelihood
+
+        def collecting(model, step):
+            theta = model.restrict(step.theta)
+            if sur.collect_llk(theta, step.data, model.precision) == 0:
+                model.source.data_llk(theta, step.data, model.precision)
+            return strategy
+
+        strategy.dataLikelihood = collecting
+        try:
+            stats = super().walkChains(annealer, step)
+        finally:
+            strategy.dataLikelihood = original
+        self.exact_evals += step.samples * self.steps
+        return stats
+
+    def _walk_da(self, annealer, step):
+        model = annealer.model
+        sur = self.surrogate
+        C, P = step.samples, step.parameters
+        dev = step.theta.device
+        seed = annealer.seed
+        stage = annealer.worker.iteration
+        f64 = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
+        ws = getattr(self, "_dws", None)
+        if ws is None or ws["cand"].shape != (C, P):
+            ws = {"cand": f64(C, P), "xi": f64(C, P), "cprior": f64(C), "cdata_s": f64(C), "cpost_s": f64(C),
+                  "post_s": f64(C), "llk_s": f64(C), "packed": f64(C, P), "cdata": f64(C),
+                  "mask": torch.empty(C, dtype=torch.int32, device=dev),
+                  "index": torch.empty(2 * C, dtype=torch.int32, device=dev),
+                  "count": torch.zeros(1, dtype=torch.int32, device=dev),
+                  "z": torch.empty((C, sur.K), dtype=torch.float32, device=dev)}
+            self._dws = ws
+        s = _lib.stream()
+        beta = float(step.beta)
+        # surrogate posterior of the current states
+        sur.data_llk(step.theta, ws["llk_s"], ws["z"])
+        _lib.call("altar_posterior", _lib.ptr(step.prior), _lib.ptr(ws["llk_s"]), beta, C, _lib.ptr(ws["post_s"]), s)
+        counters = torch.zeros(4, dtype=torch.int64, device=dev)
+        candidate = CoolingStep(beta, ws["cand"], (ws["cprior"], ws["cdata_s"], ws["cpost_s"]), sigma=step.sigma)
+        d = annealer.dispatcher
+        for k in range(self.steps):
+            d.notify(d.chainAdvanceStart, annealer)
+            _lib.call("altar_mh_propose", _lib.ptr(step.theta), C, P, _lib.ptr(self.sigma_chol), seed, stage, k,
+                      _lib.ptr(ws["xi"]), _lib.ptr(ws["cand"]), s)
+            ws["mask"].zero_()
+            model.verify(candidate, ws["mask"])
+            _lib.call("altar_mh_restore", _lib.ptr(ws["cand"]), _lib.ptr(step.theta), _lib.ptr(ws["mask"]), C, P, s)
+            model.priorLikelihood(candidate)
+            sur.data_llk(ws["cand"], ws["cdata_s"], ws["z"])
+            _lib.call("altar_da_screen", _lib.ptr(ws["cprior"]), _lib.ptr(ws["cdata_s"]), _lib.ptr(ws["post_s"]),
+                      _lib.ptr(ws["mask"]), beta, C, seed, stage, k, _lib.ptr(ws["cpost_s"]), _lib.ptr(ws["index"]),
+                      _lib.ptr(ws["count"]), _lib.ptr(counters), s)
+            n = int(ws["count"].item())
+            if n > 0:
+                _lib.call("altar_da_pack", _lib.ptr(ws["cand"]), _lib.ptr(ws["index"]), n, P, _lib.ptr(ws["packed"]),
+                          s)
+                packed = ws["packed"][:n]
+                model.source.data_llk(model.restrict(packed), ws["cdata"][:n], model.precision)
+                _lib.call("altar_da_accept", _lib.ptr(step.theta), _lib.ptr(step.prior), _lib.ptr(step.data),
+                          _lib.ptr(step.posterior), _lib.ptr(ws["post_s"]), _lib.ptr(ws["cand"]),
+                          _lib.ptr(ws["cprior"]), _lib.ptr(ws["cdata"]), _lib.ptr(ws["cpost_s"]),
+                          _lib.ptr(ws["index"]), n, beta, P, seed, stage, k, None, _lib.ptr(counters), s)
+            self.exact_evals += n
+            d.notify(d.chainAdvanceFinish, annealer)
+        acc, rej, unl2, scr = (int(v) for v in counters.cpu().tolist())
+        self.screened += scr
+        return acc, rej, unl2 + scr

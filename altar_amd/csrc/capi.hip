@@ -510,14 +510,41 @@ int altar_source_residuals(altar_source *h, const double *theta_dev, int C, doub
     return guarded([&] { forward_common(h, theta_dev, C, res_dev, ldr, precision, false, S(stream)); });
 }
 
+namespace {
+void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision, float *wr_out,
+                   size_t ldw, hipStream_t st);
+}
+
 int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
                           void *stream)
 {
+    return guarded([&] { data_llk_impl(h, theta_dev, C, llk_dev, precision, nullptr, 0, S(stream)); });
+}
+
+int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
+                                 float *wr_dev, size_t ldw, void *stream)
+{
     return guarded([&] {
+        require(wr_dev != nullptr && ldw >= h->Npad, "fields: output row pitch must be >= the padded N");
+        require(h->wkind == ALTAR_WHITEN_LOWER && h->model != ALTAR_LINEAR,
+                "fields: a full whitener and a Mogi or CDM source are required");
+        require(precision == ALTAR_FP32 || precision == ALTAR_BF16 || precision == ALTAR_BF16_REFINED,
+                "fields: precision must be FP32 or BF16");
+        data_llk_impl(h, theta_dev, C, llk_dev, precision, wr_dev, ldw, S(stream));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision, float *wr_out,
+                   size_t ldw, hipStream_t stream)
+{
+    {
         need_ready(h, true);
         require(theta_dev && llk_dev && C > 0, "bad argument");
         require(precision >= ALTAR_FP64 && precision <= ALTAR_BF16_REFINED, "unknown precision");
-        hipStream_t st = S(stream);
+        hipStream_t st = stream;
         h->ensure(C);
         const size_t Np = h->Npad, Cp = h->Cpad;
         const int ntn = (int)(Np / 128);
@@ -585,6 +612,15 @@ int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, doubl
         GemmEpi ep{};
         ep.partial = h->partial.as<float>();
         ep.ntiles = ntn;
+        int epi = EPI_SUMSQ;
+        if (wr_out) {
+            // the whitened residuals are stored too; rows past C land in a
+            // workspace (the caller's buffer has C rows)
+            require(C % 128 == 0, "fields: C must be a multiple of 128");
+            ep.out = wr_out;
+            ep.ldo = ldw;
+            epi = EPI_STORE_SUMSQ;
+        }
         if (precision == ALTAR_FP32) {
             if (h->model != ALTAR_LINEAR) {
                 h->R32.reserve(Cp * Np * sizeof(float), true);
@@ -595,7 +631,7 @@ int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, doubl
             }
             Mark m(h, 1, st);
             hip_check(launch_gemm_f32(h->R32.as<float>(), Np, h->W32.as<float>(), Np, (int)Cp, (int)Np, (int)Np, 1,
-                                      EPI_SUMSQ, ep, st),
+                                      epi, ep, st),
                       "whitening gemm");
         } else {
             h->Rhi.reserve(Cp * Np * sizeof(__hip_bfloat16), true);
@@ -614,10 +650,41 @@ int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, doubl
             }
             Mark m(h, 1, st);
             hip_check(launch_gemm_bf16x3(h->Rhi.p, h->Rlo.p, Np, h->Whi.p, h->Wlo.p, Np, (int)Cp, (int)Np, (int)Np, 1,
-                                         precision == ALTAR_BF16, EPI_SUMSQ, ep, st),
+                                         precision == ALTAR_BF16, epi, ep, st),
                       "whitening gemm");
         }
         hip_check(launch_llk_reduce(h->partial.p, 0, ntn, C, h->normalization, llk_dev, st), "reduce");
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Y = A B^T (- bias per column), FP32 MFMA; M, N multiples of 128, K of 16;
+// row pitches multiples of 4 elements
+int altar_gemm_f32(const float *A_dev, size_t lda, const float *B_dev, size_t ldb, int M, int N, int K,
+                   const float *bias_dev, float *out_dev, size_t ldo, void *stream)
+{
+    return guarded([&] {
+        require(A_dev && B_dev && out_dev && M > 0 && N > 0 && K > 0, "bad argument");
+        require(M % 128 == 0 && N % 128 == 0 && K % 16 == 0, "gemm: M, N must be multiples of 128 and K of 16");
+        require(lda % 4 == 0 && ldb % 4 == 0 && lda >= (size_t)K && ldb >= (size_t)K && ldo >= (size_t)N,
+                "gemm: bad row pitch");
+        GemmEpi ep{};
+        ep.bias = bias_dev;
+        ep.out = out_dev;
+        ep.ldo = ldo;
+        hip_check(launch_gemm_f32(A_dev, lda, B_dev, ldb, M, N, K, 0, EPI_STORE, ep, S(stream)), "gemm");
+    });
+}
+
+// dst[r, :cols] = src[idx[r], :cols] for r < n (FP32 rows)
+int altar_gather_rows_f32(const float *src_dev, size_t lds, const int *idx_dev, int n, int cols, float *dst_dev,
+                          size_t ldd, void *stream)
+{
+    return guarded([&] {
+        require(src_dev && idx_dev && dst_dev && n >= 0 && cols > 0, "bad argument");
+        hip_check(launch_gather_rows_f32(src_dev, lds, idx_dev, n, cols, dst_dev, ldd, S(stream)), "gather rows");
     });
 }
 
@@ -894,6 +961,307 @@ int altar_source_timers(altar_source *h, double *ms, long long *count, int nslot
             }
             h->marks[s].clear();
         }
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ surrogate MLP
+// Dense network in -> h -> ... -> h -> out with GELU on the hidden layers,
+// BF16 weights (FP32 master copies, Adam), FP32 accumulation.
+struct altar_mlp {
+    int nin = 0, nhid = 0, nlayers = 0, nout = 0, maxB = 0;
+    int kin = 0, kout = 0;                  // padded input / output widths
+    std::vector<int> din, dout;             // padded layer widths
+    std::vector<DevBuf> W32, Wm, Wv, W16, WT16, b32, bm, bv, gW, gb;
+    std::vector<DevBuf> pre, act16, actT16, dY16, dYT16;
+    DevBuf x16, xT16, out32, lo, hi, loss, tgt, wk;
+    long step = 0;
+    int P = 0;
+};
+
+namespace {
+
+// B input rows; the GEMMs run on B rounded up to the 64-row tile (rows past
+// B hold whatever the buffers held and are never read back)
+void mlp_forward(altar_mlp *h, const double *theta, int B, hipStream_t st)
+{
+    hip_check(launch_mlp_input(theta, B, h->P, h->lo.as<double>(), h->hi.as<double>(), h->x16.p, h->kin, st),
+              "mlp input");
+    B = (int)round_up((size_t)B, 64);
+    const void *X = h->x16.p;
+    int ldx = h->kin;
+    for (int l = 0; l < h->nlayers; ++l) {
+        MlpEpi ep{};
+        ep.bias = h->b32[l].as<float>();
+        ep.pre = h->pre[l].as<float>();
+        ep.out16 = h->act16[l].as<__bf16>();
+        ep.out32 = (l == h->nlayers - 1) ? h->out32.as<float>() : nullptr;
+        ep.ldo = h->dout[l];
+        ep.act = l < h->nlayers - 1;
+        hip_check(launch_mlp_gemm(MLP_EPI_BIAS_ACT, X, ldx, h->W16[l].p, h->din[l], B, h->dout[l], h->din[l], ep, st),
+                  "mlp gemm");
+        X = h->act16[l].p;
+        ldx = h->dout[l];
+    }
+}
+
+void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int valid, hipStream_t st)
+{
+    const int L = h->nlayers;
+    hip_check(launch_mse_grad(h->out32.as<float>(), target, w, B, h->dout[L - 1], h->dout[L - 1], valid,
+                              h->dY16[L - 1].p, h->loss.as<float>(), st),
+              "mse grad");
+    for (int l = L - 1; l >= 0; --l) {
+        const int O = h->dout[l], K = h->din[l];
+        // weight gradient: dW (O x K) = dY^T (O x B) . X (B x K) -> NT with X^T (K x B)
+        hip_check(launch_transpose_bf16(h->dY16[l].p, B, O, O, h->dYT16[l].p, B, st), "transpose");
+        const void *X = l == 0 ? h->x16.p : h->act16[l - 1].p;
+        void *XT = l == 0 ? h->xT16.p : h->actT16[l - 1].p;
+        hip_check(launch_transpose_bf16(X, B, K, K, XT, B, st), "transpose");
+        MlpEpi ep{};
+        ep.out32 = h->gW[l].as<float>();
+        ep.ldo = K;
+        hip_check(launch_mlp_gemm(MLP_EPI_F32, h->dYT16[l].p, B, XT, B, O, K, B, ep, st), "mlp dW");
+        hip_check(launch_colsum(h->dY16[l].p, B, O, O, h->gb[l].as<float>(), st), "mlp db");
+        if (l > 0) {
+            // data gradient through the previous GELU: dX = (dY . W) * gelu'(pre)
+            MlpEpi ed{};
+            ed.pre = h->pre[l - 1].as<float>();
+            ed.out16 = h->dY16[l - 1].as<__bf16>();
+            ed.ldo = K;
+            hip_check(launch_mlp_gemm(MLP_EPI_DGELU, h->dY16[l].p, O, h->WT16[l].p, O, B, K, O, ed, st), "mlp dX");
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int altar_mlp_create(altar_mlp **out, int nin, int nhid, int nlayers, int nout, int max_batch)
+{
+    return guarded([&] {
+        require(out && nin > 0 && nhid > 0 && nlayers >= 2 && nout > 0 && max_batch > 0, "bad argument");
+        require(nhid % 64 == 0, "hidden width must be a multiple of 64");
+        altar_mlp *h = new altar_mlp();
+        h->nin = nin; h->nhid = nhid; h->nlayers = nlayers; h->nout = nout;
+        h->maxB = (int)round_up((size_t)max_batch, 64);
+        h->kin = (int)round_up((size_t)nin, 64);
+        h->kout = (int)round_up((size_t)nout, 64);
+        h->P = nin;
+        for (int l = 0; l < nlayers; ++l) {
+            h->din.push_back(l == 0 ? h->kin : nhid);
+            h->dout.push_back(l == nlayers - 1 ? h->kout : nhid);
+        }
+        const int L = nlayers;
+        h->W32.resize(L); h->Wm.resize(L); h->Wv.resize(L); h->W16.resize(L); h->WT16.resize(L);
+        h->b32.resize(L); h->bm.resize(L); h->bv.resize(L); h->gW.resize(L); h->gb.resize(L);
+        h->pre.resize(L); h->act16.resize(L); h->actT16.resize(L); h->dY16.resize(L); h->dYT16.resize(L);
+        const size_t B = h->maxB;
+        for (int l = 0; l < L; ++l) {
+            const size_t O = h->dout[l], K = h->din[l];
+            h->W32[l].reserve(O * K * 4, true); h->Wm[l].reserve(O * K * 4, true); h->Wv[l].reserve(O * K * 4, true);
+            h->W16[l].reserve(O * K * 2, true); h->WT16[l].reserve(O * K * 2, true);
+            h->b32[l].reserve(O * 4, true); h->bm[l].reserve(O * 4, true); h->bv[l].reserve(O * 4, true);
+            h->gW[l].reserve(O * K * 4, true); h->gb[l].reserve(O * 4, true);
+            h->pre[l].reserve(B * O * 4, true); h->act16[l].reserve(B * O * 2, true);
+            h->actT16[l].reserve(B * O * 2, true);
+            h->dY16[l].reserve(B * O * 2, true); h->dYT16[l].reserve(B * O * 2, true);
+        }
+        h->x16.reserve(B * h->kin * 2, true);
+        h->xT16.reserve(B * h->kin * 2, true);
+        h->out32.reserve(B * h->kout * 4, true);
+        h->lo.reserve(nin * 8, true);
+        h->hi.reserve(nin * 8, true);
+        h->loss.reserve(4, true);
+        h->tgt.reserve(B * h->kout * 4, true);
+        h->wk.reserve(h->kout * 4, true);
+        *out = h;
+    });
+}
+
+int altar_mlp_destroy(altar_mlp *h)
+{
+    return guarded([&] {
+        if (!h) return;
+        for (auto *v : {&h->W32, &h->Wm, &h->Wv, &h->W16, &h->WT16, &h->b32, &h->bm, &h->bv, &h->gW, &h->gb,
+                        &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16})
+            for (auto &b : *v) b.release();
+        for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk}) b->release();
+        delete h;
+    });
+}
+
+int altar_mlp_input_scaling(altar_mlp *h, const double *lo, const double *hi)
+{
+    return guarded([&] {
+        require(h && lo && hi, "bad argument");
+        for (int j = 0; j < h->nin; ++j) require(hi[j] > lo[j], "input scaling: empty range");
+        hip_check(hipMemcpy(h->lo.p, lo, h->nin * 8, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(h->hi.p, hi, h->nin * 8, hipMemcpyHostToDevice), "H2D");
+    });
+}
+
+// layer l weights (out x in, row-major, unpadded) and biases (out), FP32 host
+int altar_mlp_set_layer(altar_mlp *h, int l, const float *W, const float *b)
+{
+    return guarded([&] {
+        require(h && W && b && l >= 0 && l < h->nlayers, "bad argument");
+        const int O = h->dout[l], K = h->din[l];
+        const int o_real = l == h->nlayers - 1 ? h->nout : h->nhid;
+        const int k_real = l == 0 ? h->nin : h->nhid;
+        std::vector<float> w(O * K, 0.0f), bb(O, 0.0f);
+        std::vector<__hip_bfloat16> w16(O * K, to_bf16(0.0f)), wt16(O * K, to_bf16(0.0f));
+        for (int o = 0; o < o_real; ++o) {
+            for (int k = 0; k < k_real; ++k) {
+                w[o * K + k] = W[o * k_real + k];
+                w16[o * K + k] = to_bf16(W[o * k_real + k]);
+                wt16[k * O + o] = to_bf16(W[o * k_real + k]);
+            }
+            bb[o] = b[o];
+        }
+        hip_check(hipMemcpy(h->W32[l].p, w.data(), w.size() * 4, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(h->W16[l].p, w16.data(), w16.size() * 2, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(h->WT16[l].p, wt16.data(), wt16.size() * 2, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(h->b32[l].p, bb.data(), bb.size() * 4, hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemset(h->Wm[l].p, 0, w.size() * 4), "memset");
+        hip_check(hipMemset(h->Wv[l].p, 0, w.size() * 4), "memset");
+        hip_check(hipMemset(h->bm[l].p, 0, bb.size() * 4), "memset");
+        hip_check(hipMemset(h->bv[l].p, 0, bb.size() * 4), "memset");
+        h->step = 0;
+    });
+}
+
+// read back layer l: FP32 master weights and biases (unpadded); with grads != 0
+// the last computed gradients instead
+int altar_mlp_get_layer(altar_mlp *h, int l, float *W, float *b, int grads)
+{
+    return guarded([&] {
+        require(h && W && b && l >= 0 && l < h->nlayers, "bad argument");
+        hip_check(hipDeviceSynchronize(), "sync");
+        const int O = h->dout[l], K = h->din[l];
+        const int o_real = l == h->nlayers - 1 ? h->nout : h->nhid;
+        const int k_real = l == 0 ? h->nin : h->nhid;
+        std::vector<float> w(O * K), bb(O);
+        hip_check(hipMemcpy(w.data(), grads ? h->gW[l].p : h->W32[l].p, w.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(bb.data(), grads ? h->gb[l].p : h->b32[l].p, bb.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        for (int o = 0; o < o_real; ++o) {
+            for (int k = 0; k < k_real; ++k) W[o * k_real + k] = w[o * K + k];
+            b[o] = bb[o];
+        }
+    });
+}
+
+// z_dev (B x ldz, FP32) = network(theta_dev); B <= max_batch
+int altar_mlp_forward(altar_mlp *h, const double *theta_dev, int B, float *z_dev, int ldz, void *stream)
+{
+    return guarded([&] {
+        require(h && theta_dev && z_dev && B > 0 && B <= h->maxB && ldz >= h->nout, "bad argument");
+        hipStream_t st = S(stream);
+        mlp_forward(h, theta_dev, B, st);
+        hip_check(hipMemcpy2DAsync(z_dev, ldz * 4, h->out32.p, h->kout * 4, h->nout * 4, B, hipMemcpyDeviceToDevice, st),
+                  "copy out");
+    });
+}
+
+// gradients of the weighted squared error over a batch (no update):
+// loss = sum_b sum_k w_k (net(theta_b)_k - target_bk)^2 / B; target is B x nout
+// (FP32, device, row pitch ldt); loss_dev (optional) receives the loss
+int altar_mlp_backward(altar_mlp *h, const double *theta_dev, const float *target_dev, int ldt, const float *w_dev,
+                       int B, float *loss_dev, void *stream)
+{
+    return guarded([&] {
+        require(h && theta_dev && target_dev && w_dev && B > 0 && B <= h->maxB, "bad argument");
+        hipStream_t st = S(stream);
+        const int Bp = (int)round_up((size_t)B, 64);
+        // rows B..Bp-1 of the input are zeroed, their targets are ignored
+        hip_check(hipMemsetAsync(h->x16.p, 0, (size_t)h->maxB * h->kin * 2, st), "memset");
+        mlp_forward(h, theta_dev, B, st);
+        // targets and weights into the padded (maxB x kout) layout
+        hip_check(hipMemsetAsync(h->tgt.p, 0, (size_t)h->maxB * h->kout * 4, st), "memset");
+        hip_check(hipMemsetAsync(h->wk.p, 0, (size_t)h->kout * 4, st), "memset");
+        hip_check(hipMemcpy2DAsync(h->tgt.p, h->kout * 4, target_dev, ldt * 4, h->nout * 4, B,
+                                   hipMemcpyDeviceToDevice, st),
+                  "targets");
+        hip_check(hipMemcpyAsync(h->wk.p, w_dev, h->nout * 4, hipMemcpyDeviceToDevice, st), "weights");
+        hip_check(hipMemsetAsync(h->loss.p, 0, 4, st), "memset");
+        mlp_backward(h, h->tgt.as<float>(), h->wk.as<float>(), Bp, B, st);
+        if (loss_dev) hip_check(hipMemcpyAsync(loss_dev, h->loss.p, 4, hipMemcpyDeviceToDevice, st), "loss");
+    });
+}
+
+// one Adam step on the gradients of the last altar_mlp_backward
+int altar_mlp_adam(altar_mlp *h, float lr, float beta1, float beta2, float eps, void *stream)
+{
+    return guarded([&] {
+        require(h && lr > 0, "bad argument");
+        hipStream_t st = S(stream);
+        h->step += 1;
+        const float c1 = 1.0f - std::pow(beta1, (float)h->step), c2 = 1.0f - std::pow(beta2, (float)h->step);
+        for (int l = 0; l < h->nlayers; ++l) {
+            const int O = h->dout[l], K = h->din[l];
+            hip_check(launch_adam(h->W32[l].as<float>(), h->Wm[l].as<float>(), h->Wv[l].as<float>(),
+                                  h->gW[l].as<float>(), O, K, lr, beta1, beta2, eps, c1, c2, h->W16[l].p,
+                                  h->WT16[l].p, st),
+                      "adam W");
+            hip_check(launch_adam(h->b32[l].as<float>(), h->bm[l].as<float>(), h->bv[l].as<float>(),
+                                  h->gb[l].as<float>(), 1, O, lr, beta1, beta2, eps, c1, c2, nullptr, nullptr, st),
+                      "adam b");
+        }
+    });
+}
+
+// surrogate data llk from predicted normalized coefficients z (C x ldz):
+// llk = norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2 (e, s: K values, device FP64)
+int altar_surrogate_llk(const float *z_dev, int ldz, int C, int K, const double *e_dev, const double *s_dev,
+                        double cperp, double normalization, double *llk_dev, void *stream)
+{
+    return guarded([&] {
+        require(z_dev && e_dev && s_dev && llk_dev && C > 0 && K > 0 && ldz >= K, "bad argument");
+        hip_check(launch_surrogate_llk(z_dev, ldz, C, K, e_dev, s_dev, cperp, normalization, llk_dev, S(stream)),
+                  "surrogate llk");
+    });
+}
+
+// ------------------------------------------------------------------ delayed acceptance
+int altar_da_screen(const double *cprior_dev, const double *cdata_s_dev, const double *post_s_dev,
+                    const int *mask_dev, double beta, int C, uint64_t seed, uint32_t stage, uint32_t step,
+                    double *cpost_s_dev, int *index_dev, int *count_dev, unsigned long long *counters_dev,
+                    void *stream)
+{
+    return guarded([&] {
+        require(cprior_dev && cdata_s_dev && post_s_dev && mask_dev && cpost_s_dev && index_dev && count_dev &&
+                    counters_dev && C > 0,
+                "bad argument");
+        hip_check(launch_da_screen(cprior_dev, cdata_s_dev, post_s_dev, mask_dev, beta, C, seed, stage, step,
+                                   cpost_s_dev, index_dev, count_dev, counters_dev, S(stream)),
+                  "da screen");
+    });
+}
+
+int altar_da_pack(const double *cand_dev, const int *index_dev, int n, int P, double *packed_dev, void *stream)
+{
+    return guarded([&] {
+        require(cand_dev && index_dev && packed_dev && n >= 0 && P > 0, "bad argument");
+        hip_check(launch_da_pack(cand_dev, index_dev, n, P, packed_dev, S(stream)), "da pack");
+    });
+}
+
+int altar_da_accept(double *theta_dev, double *prior_dev, double *data_dev, double *post_dev, double *post_s_dev,
+                    const double *cand_dev, const double *cprior_dev, const double *cdata_packed_dev,
+                    const double *cpost_s_dev, const int *index_dev, int n, double beta, int P, uint64_t seed,
+                    uint32_t stage, uint32_t step, int *decision_dev, unsigned long long *counters_dev,
+                    void *stream)
+{
+    return guarded([&] {
+        require(theta_dev && prior_dev && data_dev && post_dev && post_s_dev && cand_dev && cprior_dev &&
+                    cdata_packed_dev && cpost_s_dev && index_dev && counters_dev && n >= 0 && P > 0,
+                "bad argument");
+        hip_check(launch_da_accept(theta_dev, prior_dev, data_dev, post_dev, post_s_dev, cand_dev, cprior_dev,
+                                   cdata_packed_dev, cpost_s_dev, index_dev, n, beta, P, seed, stage, step,
+                                   decision_dev, counters_dev, S(stream)),
+                  "da accept");
     });
 }
 

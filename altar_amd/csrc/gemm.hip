@@ -59,6 +59,19 @@ __device__ inline void gemm_epilogue(floatx16 (&acc)[2][2], const GemmEpi &ep, i
                 }
             }
     } else {
+        if constexpr (EPI == EPI_STORE_SUMSQ) {
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) {
+                    const int col = n0 + wn * 64 + ni * 32 + (lane & 31);
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int row = m0 + wm * 64 + mi * 32 + acc_row(v, lane);
+                        ep.out[(size_t)row * ep.ldo + col] = acc[mi][ni][v];
+                    }
+                }
+        }
         // sum over this block's 128 columns of (Y - bias)^2, per row
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi) {
@@ -349,6 +362,8 @@ hipError_t launch_gemm_f32(const float *A, size_t lda, const float *B, size_t ld
     dim3 grid(Np / BN, Mp / BM), block(256);
     if (epi == EPI_STORE)
         hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+    else if (epi == EPI_STORE_SUMSQ)
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
     else if (epi == EPI_SUMSQ)
         hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
     else
@@ -365,10 +380,31 @@ hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, cons
     const __bf16 *bh = (const __bf16 *)Bhi, *bl = (const __bf16 *)Blo;
     if (epi == EPI_STORE)
         hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+    else if (epi == EPI_STORE_SUMSQ)
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
     else if (epi == EPI_SUMSQ)
         hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
     else
         hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ_BIAS>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+    return hipGetLastError();
+}
+
+__global__ void gather_rows_f32_kernel(const float *__restrict__ src, size_t lds, const int *__restrict__ idx, int n,
+                                       int cols, float *__restrict__ dst, size_t ldd)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * cols) return;
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    dst[(size_t)r * ldd + c] = src[(size_t)idx[r] * lds + c];
+}
+
+hipError_t launch_gather_rows_f32(const float *src, size_t lds, const int *idx, int n, int cols, float *dst,
+                                  size_t ldd, hipStream_t st)
+{
+    long tot = (long)n * cols;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_f32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, src, lds, idx,
+                       n, cols, dst, ldd);
     return hipGetLastError();
 }
 

@@ -16,7 +16,7 @@ constexpr int MOGI_PRM = 4 + MAX_SETS;
 // forward-kernel epilogues
 enum { MODE_DIAG = 0, MODE_RES_F32 = 1, MODE_RES_F64 = 2, MODE_RES_BF16 = 3 };
 // GEMM epilogues
-enum { EPI_STORE = 0, EPI_SUMSQ = 1, EPI_SUMSQ_BIAS = 2 };
+enum { EPI_STORE = 0, EPI_SUMSQ = 1, EPI_SUMSQ_BIAS = 2, EPI_STORE_SUMSQ = 3 };
 
 // observations, structure of arrays, FP32 and FP64 copies
 struct ObsView {
@@ -68,6 +68,18 @@ struct GemmEpi {
     int ntiles;
 };
 
+// surrogate MLP GEMM epilogues
+enum { MLP_EPI_BIAS_ACT = 0, MLP_EPI_F32 = 1, MLP_EPI_DGELU = 2 };
+
+struct MlpEpi {
+    const float *bias;
+    float *pre;           // BIAS_ACT: pre-activation out; DGELU: pre-activation in
+    float *out32;
+    __bf16 *out16;
+    int ldo;
+    int act;              // BIAS_ACT: apply GELU
+};
+
 struct AcceptArgs {
     double *theta, *prior, *data, *post;
     const double *ctheta, *cprior, *cdata;
@@ -94,6 +106,8 @@ hipError_t launch_gemm_f32(const float *A, size_t lda, const float *B, size_t ld
 hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, const void *Bhi, const void *Blo,
                               size_t ldb, int Mp, int Np, int Kp, int tri, int plain, int epi,
                               const GemmEpi &ep, hipStream_t st);
+hipError_t launch_gather_rows_f32(const float *src, size_t lds, const int *idx, int n, int cols, float *dst,
+                                  size_t ldd, hipStream_t st);
 hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
                              double *llk, hipStream_t st);
 hipError_t launch_whiten_f64(const double *R, size_t ldr, const double *W, size_t ldw, int N, int C,
@@ -119,6 +133,31 @@ hipError_t launch_wcov(const double *theta, const double *w, int C, int P, doubl
 hipError_t launch_gather(const double *theta_old, const double *prior_old, const double *data_old,
                          const double *post_old, const int *src, int C, int P, double *theta, double *prior,
                          double *data, double *post, hipStream_t st);
+// mlp.hip
+hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
+                           const MlpEpi &ep, hipStream_t st);
+hipError_t launch_transpose_bf16(const void *in, int rows, int cols, int ldi, void *out, int ldo, hipStream_t st);
+hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, int rows, int cols, int ld,
+                           int valid_rows, void *dy, float *loss, hipStream_t st);
+hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *db, hipStream_t st);
+hipError_t launch_adam(float *w, float *m, float *v, const float *g, int rows, int cols, float lr, float b1,
+                       float b2, float eps, float c1, float c2, void *w16, void *wt16, hipStream_t st);
+hipError_t launch_surrogate_llk(const float *z, int ldz, int C, int K, const double *e, const double *s,
+                                double cperp, double norm, double *llk, hipStream_t st);
+hipError_t launch_mlp_input(const double *theta, int C, int P, const double *lo, const double *hi, void *x, int ldx,
+                            hipStream_t st);
+
+// delayed acceptance (sampler.hip)
+hipError_t launch_da_screen(const double *cprior, const double *cdata_s, const double *post_s, const int *mask,
+                            double beta, int C, uint64_t seed, uint32_t stage, uint32_t step, double *cpost_s,
+                            int *index, int *count, unsigned long long *counters, hipStream_t st);
+hipError_t launch_da_pack(const double *cand, const int *index, int n, int P, double *packed, hipStream_t st);
+hipError_t launch_da_accept(double *theta, double *prior, double *data, double *post, double *post_s,
+                            const double *cand, const double *cprior, const double *cdata_packed,
+                            const double *cpost_s, const int *index, int n, double beta, int P, uint64_t seed,
+                            uint32_t stage, uint32_t step, int *decision, unsigned long long *counters,
+                            hipStream_t st);
+
 hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int *mask, hipStream_t st);
 
 }  // namespace altar

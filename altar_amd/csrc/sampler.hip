@@ -306,3 +306,161 @@ hipError_t launch_gather(const double *theta_old, const double *prior_old, const
 }
 
 }  // namespace altar
+
+namespace altar {
+
+// ---------------------------------------------------------------- delayed acceptance
+// Two-stage Metropolis (Christen and Fox 2005).  Stage 1 screens the
+// candidate with the surrogate posterior cpost_s = cprior + beta cdata_s:
+// pass if the model accepts it and log u1 <= cpost_s - post_s (u1 from the
+// screening stream).  A second kernel lists the passing chains in chain
+// order (a one-block ballot scan), so the list does not depend on
+// scheduling.
+__global__ void da_screen_kernel(const double *__restrict__ cprior, const double *__restrict__ cdata_s,
+                                 const double *__restrict__ post_s, const int *__restrict__ mask, double beta,
+                                 int C, uint64_t seed, uint32_t stage, uint32_t step, double *__restrict__ cpost_s,
+                                 int *__restrict__ flag, unsigned long long *__restrict__ counters)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int pass = 0, rej = 0, scr = 0;
+    if (c < C) {
+        const double cp = __dadd_rn(cprior[c], __dmul_rn(beta, cdata_s[c]));
+        cpost_s[c] = cp;
+        if (mask[c]) {
+            rej = 1;
+        } else {
+            u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_SCREEN, 0u);
+            const double u = u53(r.x, r.y);
+            if (log(u) <= __dadd_rn(cp, -post_s[c])) pass = 1;
+            else scr = 1;
+        }
+        flag[c] = pass;
+    }
+    __shared__ unsigned cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    if (rej) atomicAdd(&cnt[0], 1u);
+    if (scr) atomicAdd(&cnt[1], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (cnt[0]) atomicAdd(&counters[1], (unsigned long long)cnt[0]);
+        if (cnt[1]) atomicAdd(&counters[3], (unsigned long long)cnt[1]);
+    }
+}
+
+// exclusive scan of the pass flags in one block of 1024 threads (C chains in
+// chunks), writing the ordered list of passing chains and their count
+__global__ void da_compact_kernel(const int *__restrict__ flag, int C, int *__restrict__ index,
+                                  int *__restrict__ count)
+{
+    __shared__ int wsum[16];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int start = 0; start < C; start += blockDim.x) {
+        const int c = start + threadIdx.x;
+        const int f = c < C ? flag[c] : 0;
+        const unsigned long long bal = __ballot(f);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wid] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wid; ++w) off += wsum[w];
+        if (f) index[off + before] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += wsum[w];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+// packed[j] = cand[index[j]] for the n screened-in chains
+__global__ void da_pack_kernel(const double *__restrict__ cand, const int *__restrict__ index, int n, int P,
+                               double *__restrict__ packed)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * P) return;
+    const int j = (int)(idx / P), k = (int)(idx % P);
+    packed[idx] = cand[(size_t)index[j] * P + k];
+}
+
+// Stage 2 for the n screened-in chains: accept if
+// log u2 <= (cpost - post) - (cpost_s - post_s) with the exact cpost.
+// decision (optional, per chain): 0 accepted, 2 unlikely at stage 2.
+__global__ void da_accept_kernel(double *__restrict__ theta, double *__restrict__ prior, double *__restrict__ data,
+                                 double *__restrict__ post, double *__restrict__ post_s,
+                                 const double *__restrict__ cand, const double *__restrict__ cprior,
+                                 const double *__restrict__ cdata_packed, const double *__restrict__ cpost_s,
+                                 const int *__restrict__ index, int n, double beta, int P, uint64_t seed,
+                                 uint32_t stage, uint32_t step, int *__restrict__ decision,
+                                 unsigned long long *__restrict__ counters)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int acc = 0, unl = 0;
+    if (j < n) {
+        const int c = index[j];
+        const double cp = __dadd_rn(cprior[c], __dmul_rn(beta, cdata_packed[j]));
+        u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_ACCEPT, 0u);
+        const double u = u53(r.x, r.y);
+        const double lhs = __dadd_rn(__dadd_rn(cp, -post[c]), -__dadd_rn(cpost_s[c], -post_s[c]));
+        if (log(u) <= lhs) {
+            acc = 1;
+            for (int k = 0; k < P; ++k) theta[(size_t)c * P + k] = cand[(size_t)c * P + k];
+            prior[c] = cprior[c];
+            data[c] = cdata_packed[j];
+            post[c] = cp;
+            post_s[c] = cpost_s[c];
+        } else {
+            unl = 1;
+        }
+        if (decision) decision[c] = acc ? 0 : 2;
+    }
+    __shared__ unsigned cnt[2];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    if (acc) atomicAdd(&cnt[0], 1u);
+    if (unl) atomicAdd(&cnt[1], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (cnt[0]) atomicAdd(&counters[0], (unsigned long long)cnt[0]);
+        if (cnt[1]) atomicAdd(&counters[2], (unsigned long long)cnt[1]);
+    }
+}
+
+hipError_t launch_da_screen(const double *cprior, const double *cdata_s, const double *post_s, const int *mask,
+                            double beta, int C, uint64_t seed, uint32_t stage, uint32_t step, double *cpost_s,
+                            int *index, int *count, unsigned long long *counters, hipStream_t st)
+{
+    // the pass flags are staged in the upper half of `index` (2 C ints)
+    int *flag = index + C;
+    hipLaunchKernelGGL(da_screen_kernel, grid1(C), dim3(256), 0, st, cprior, cdata_s, post_s, mask, beta, C, seed,
+                       stage, step, cpost_s, flag, counters);
+    hipLaunchKernelGGL(da_compact_kernel, dim3(1), dim3(1024), 0, st, flag, C, index, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_da_pack(const double *cand, const int *index, int n, int P, double *packed, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(da_pack_kernel, grid1((long)n * P), dim3(256), 0, st, cand, index, n, P, packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_da_accept(double *theta, double *prior, double *data, double *post, double *post_s,
+                            const double *cand, const double *cprior, const double *cdata_packed,
+                            const double *cpost_s, const int *index, int n, double beta, int P, uint64_t seed,
+                            uint32_t stage, uint32_t step, int *decision, unsigned long long *counters,
+                            hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(da_accept_kernel, grid1(n), dim3(256), 0, st, theta, prior, data, post, post_s, cand, cprior,
+                       cdata_packed, cpost_s, index, n, beta, P, seed, stage, step, decision, counters);
+    return hipGetLastError();
+}
+
+}  // namespace altar

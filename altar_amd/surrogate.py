@@ -1,0 +1,264 @@
+"""Learned surrogate of the forward model for delayed-acceptance screening.
+
+No reference counterpart: this is the north-star addition to CATMIP.  A
+dense network (10 -> 256 -> 256 -> 256 -> 64, GELU, BF16 weights, FP32
+accumulation; kernels in csrc/mlp.hip) maps source parameters to the leading
+principal components of the *whitened* residual field W r(theta), where W is
+the likelihood's whitener.  In that space the surrogate log-likelihood is a
+64-term sum:
+
+    |W r|^2 ~= cperp + sum_k (e_k + s_k z_k)^2,
+    e = V^T mean,  cperp = |(I - V V^T) mean|^2,
+
+with V the orthonormal PCA basis, ``mean`` the mean whitened residual of the
+training set, s the coefficient scales and z the network output.  The
+training fields are the whitened residuals that the annealer's own exact
+evaluations produce (altar_source_data_llk_fields), so no extra forward
+model runs are needed to build the training set.
+
+The network is trained on the GPU by its own forward and backward kernels
+(Adam).  Validation reports the RMSE of the predicted LOS field in metres on
+held-out samples; the sampler falls back to the exact path above a bound.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class MLP:
+    """Handle of an altar_mlp network."""
+
+    def __init__(self, nin, nhid=256, nlayers=4, nout=64, max_batch=16384):
+        h = ctypes.c_void_p()
+        _lib.call("altar_mlp_create", ctypes.byref(h), nin, nhid, nlayers, nout, max_batch)
+        self.handle = h
+        self.nin, self.nhid, self.nlayers, self.nout, self.max_batch = nin, nhid, nlayers, nout, max_batch
+        self.shapes = [((nhid if l < nlayers - 1 else nout), (nin if l == 0 else nhid)) for l in range(nlayers)]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().altar_mlp_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def init(self, seed=0):
+        """He-normal weights (GELU layers), LeCun-normal output layer, zero biases."""
+        rng = np.random.default_rng(seed)
+        for l, (o, k) in enumerate(self.shapes):
+            gain = 2.0 if l < self.nlayers - 1 else 1.0
+            self.set_layer(l, rng.normal(0.0, np.sqrt(gain / k), (o, k)), np.zeros(o))
+        return self
+
+    def input_scaling(self, lo, hi):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        _lib.call("altar_mlp_input_scaling", self.handle, _lib.ptr(lo), _lib.ptr(hi))
+
+    def set_layer(self, l, W, b):
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        assert W.shape == self.shapes[l] and b.shape == (self.shapes[l][0],)
+        _lib.call("altar_mlp_set_layer", self.handle, l, _lib.ptr(W), _lib.ptr(b))
+
+    def layer(self, l, grads=False):
+        W = np.empty(self.shapes[l], dtype=np.float32)
+        b = np.empty(self.shapes[l][0], dtype=np.float32)
+        _lib.call("altar_mlp_get_layer", self.handle, l, _lib.ptr(W), _lib.ptr(b), int(grads))
+        return W, b
+
+    def forward(self, theta, out=None):
+        theta = theta.contiguous()
+        B = theta.shape[0]
+        if out is None:
+            out = torch.empty((B, self.nout), dtype=torch.float32, device=theta.device)
+        _lib.call("altar_mlp_forward", self.handle, _lib.ptr(theta), B, _lib.ptr(out), out.shape[1], _lib.stream())
+        return out
+
+    def backward(self, theta, target, weights, loss=None):
+        theta = theta.contiguous()
+        _lib.call("altar_mlp_backward", self.handle, _lib.ptr(theta), _lib.ptr(target), target.shape[1],
+                  _lib.ptr(weights), theta.shape[0], _lib.ptr(loss), _lib.stream())
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _lib.call("altar_mlp_adam", self.handle, float(lr), float(beta1), float(beta2), float(eps), _lib.stream())
+
+
+def whitened_pca(wr, K, oversample=32, iterations=3, seed=0):
+    """Mean and top-K right singular vectors of the rows of ``wr`` (host,
+    randomized SVD with power iterations, Halko, Martinsson and Tropp 2011)."""
+    wr = np.asarray(wr, dtype=np.float64)
+    mean = wr.mean(axis=0)
+    X = wr - mean
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(X @ rng.standard_normal((X.shape[1], K + oversample)))[0]
+    for _ in range(iterations):
+        Q = np.linalg.qr(X.T @ Q)[0]
+        Q = np.linalg.qr(X @ Q)[0]
+    _, sv, Vt = np.linalg.svd(Q.T @ X, full_matrices=False)
+    return mean, Vt[:K].T.copy(), sv[:K]
+
+
+class Surrogate:
+    """PCA + MLP surrogate of a source's whitened residual field."""
+
+    def __init__(self, model, K=64, hidden=256, layers=4, capacity=200_000, max_batch=16384, seed=0):
+        self.model = model
+        self.K = K
+        self.P = model.parameters
+        self.N = model.observations
+        self.Npad = (self.N + 127) // 128 * 128
+        self.capacity = (int(capacity) + 127) // 128 * 128
+        self.max_batch = max_batch
+        self.seed = seed
+        dev = model.device
+        self.theta = torch.empty((self.capacity, self.P), dtype=torch.float64, device=dev)
+        self.wr = torch.empty((self.capacity, self.Npad), dtype=torch.float32, device=dev)
+        self.count = 0
+        self.mlp = MLP(self.P, hidden, layers, K, max_batch).init(seed)
+        from .distributions import flatten
+        kinds, lo, hi = flatten(model.psets, "prior")
+        self.lo, self.hi = np.array(lo), np.array(hi)
+        self.mlp.input_scaling(self.lo, self.hi)
+        self.trained = False
+        self.validation_rmse = None
+        self.history = []
+
+    # -- training data
+    def space(self):
+        return self.capacity - self.count
+
+    def collect_llk(self, theta, llk_out, precision="fp32"):
+        """Exact data llk of ``theta`` (C rows, C % 128 == 0) that also appends the
+        whitened residuals to the training set.  Returns the number stored."""
+        C = theta.shape[0]
+        if self.space() < C:
+            self.model.source.data_llk(theta, llk_out, precision)
+            return 0
+        theta = theta.contiguous()
+        dst = self.wr[self.count:self.count + C]
+        _lib.call("altar_source_data_llk_fields", self.model.source.handle, _lib.ptr(theta), C, _lib.ptr(llk_out),
+                  _lib.PRECISIONS[precision], _lib.ptr(dst), self.Npad, _lib.stream())
+        self.theta[self.count:self.count + C].copy_(theta)
+        self.count += C
+        return C
+
+    # -- fitting
+    def fit(self, epochs=40, batch=4096, lr=2e-3, holdout=0.05, pca_sample=40_000, lr_final=1e-4):
+        """PCA of the collected whitened residuals, then Adam on the MLP."""
+        n = self.count
+        assert n >= 1024, "too few training fields"
+        dev = self.theta.device
+        rng = np.random.default_rng(self.seed)
+        perm = rng.permutation(n)
+        nval = max(256, int(holdout * n))
+        val_idx, tr_idx = perm[:nval], perm[nval:]
+        # PCA on a host sample of the training rows (whitened residuals)
+        samp = np.sort(rng.choice(tr_idx, size=min(pca_sample, len(tr_idx)), replace=False))
+        idx_dev = torch.from_numpy(samp.astype(np.int32)).to(dev)
+        tmp = torch.empty((len(samp), self.N), dtype=torch.float32, device=dev)
+        _lib.call("altar_gather_rows_f32", _lib.ptr(self.wr), self.Npad, _lib.ptr(idx_dev), len(samp), self.N,
+                  _lib.ptr(tmp), self.N, _lib.stream())
+        mean, V, sv = whitened_pca(tmp.cpu().numpy(), self.K, seed=self.seed)
+        del tmp
+        self.mean, self.V = mean, V
+        self.e = V.T @ mean
+        self.cperp = float(mean @ mean - self.e @ self.e)
+        # coefficients of every stored field: c = V^T (wr - mean), FP32 MFMA GEMM
+        Kp = 128
+        Vt = np.zeros((Kp, self.Npad), dtype=np.float32)
+        Vt[:self.K, :self.N] = V.T
+        bias = np.zeros(Kp, dtype=np.float32)
+        bias[:self.K] = self.e
+        Vt_d = torch.from_numpy(Vt).to(dev)
+        bias_d = torch.from_numpy(bias).to(dev)
+        rows = (n + 127) // 128 * 128
+        coef = torch.zeros((rows, Kp), dtype=torch.float32, device=dev)
+        _lib.call("altar_gemm_f32", _lib.ptr(self.wr), self.Npad, _lib.ptr(Vt_d), self.Npad, rows, Kp, self.Npad,
+                  _lib.ptr(bias_d), _lib.ptr(coef), Kp, _lib.stream())
+        c_host = coef[:n, :self.K].cpu().numpy().astype(np.float64)
+        s = c_host.std(axis=0) + 1e-12
+        self.s = s
+        target_all = torch.from_numpy((c_host / s).astype(np.float32)).to(dev)
+        del coef
+        # loss weights: the share of whitened variance of each component
+        w = s ** 2 / np.sum(s ** 2)
+        w_d = torch.from_numpy(w.astype(np.float32)).to(dev)
+        self.e_d = torch.from_numpy(self.e).to(dev)
+        self.s_d = torch.from_numpy(s).to(dev)
+        # training loop over shuffled mini-batches (gathered rows)
+        tr = np.asarray(tr_idx)
+        batch = min(batch, self.max_batch)
+        nb = max(1, len(tr) // batch)
+        xb = torch.empty((batch, self.P), dtype=torch.float64, device=dev)
+        yb = torch.empty((batch, self.K), dtype=torch.float32, device=dev)
+        loss = torch.zeros(1, dtype=torch.float32, device=dev)
+        total = epochs * nb
+        it = 0
+        for ep in range(epochs):
+            order = torch.from_numpy(rng.permutation(tr).astype(np.int32)).to(dev)
+            ep_loss = 0.0
+            for b in range(nb):
+                sel = order[b * batch:(b + 1) * batch]
+                _lib.call("altar_da_pack", _lib.ptr(self.theta), _lib.ptr(sel), batch, self.P, _lib.ptr(xb),
+                          _lib.stream())
+                _lib.call("altar_gather_rows_f32", _lib.ptr(target_all), self.K, _lib.ptr(sel), batch, self.K,
+                          _lib.ptr(yb), self.K, _lib.stream())
+                self.mlp.backward(xb, yb, w_d, loss)
+                # cosine decay from lr to lr_final
+                frac = it / max(1, total - 1)
+                cur = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac))
+                self.mlp.adam(cur)
+                it += 1
+            ep_loss = float(loss.item())
+            self.history.append(ep_loss)
+        self.trained = True
+        self.val_idx = val_idx
+        self.validation_rmse = self.validate_stored(val_idx)
+        return self
+
+    # -- inference
+    def data_llk(self, theta, out, z=None):
+        z = self.mlp.forward(theta, z)
+        _lib.call("altar_surrogate_llk", _lib.ptr(z), z.shape[1], theta.shape[0], self.K, _lib.ptr(self.e_d),
+                  _lib.ptr(self.s_d), self.cperp, float(self.model.normalization), _lib.ptr(out), _lib.stream())
+        return out
+
+    def predict_wr(self, theta):
+        """Predicted whitened residuals (host, FP64) of the given samples."""
+        z = self.mlp.forward(theta).double().cpu().numpy()
+        return self.mean + (z * self.s) @ self.V.T
+
+    def field_rmse(self, wr_pred, wr_true):
+        """RMSE of the LOS field in metres: W^-1 maps whitened residuals back."""
+        L = getattr(self, "_L", None)
+        if L is None and self.model.whitening_kind == "lower":
+            # W = chol(Cd)^-1 for the exact whitener, so W^-1 = chol(Cd)
+            if getattr(self.model, "whiten", "exact") == "exact":
+                L = np.linalg.cholesky(np.asarray(self.model.cd, dtype=np.float64))
+            else:
+                L = np.linalg.inv(self.model.W)
+            self._L = L
+        diff = wr_pred - wr_true
+        if L is None:
+            f = diff / self.model.W[None, :]
+        else:
+            f = diff @ L.T
+        return float(np.sqrt(np.mean(f ** 2)))
+
+    def validate_stored(self, idx):
+        idx = np.sort(np.asarray(idx))[:4096]
+        dev = self.theta.device
+        idx_d = torch.from_numpy(idx.astype(np.int32)).to(dev)
+        th = torch.empty((len(idx), self.P), dtype=torch.float64, device=dev)
+        _lib.call("altar_da_pack", _lib.ptr(self.theta), _lib.ptr(idx_d), len(idx), self.P, _lib.ptr(th),
+                  _lib.stream())
+        wr = torch.empty((len(idx), self.N), dtype=torch.float32, device=dev)
+        _lib.call("altar_gather_rows_f32", _lib.ptr(self.wr), self.Npad, _lib.ptr(idx_d), len(idx), self.N,
+                  _lib.ptr(wr), self.N, _lib.stream())
+        return self.field_rmse(self.predict_wr(th), wr.double().cpu().numpy())

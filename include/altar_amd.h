@@ -112,6 +112,13 @@ int altar_source_residuals(altar_source *h, const double *theta_dev, int C, doub
 int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
                           void *stream);
 
+/* As altar_source_data_llk, and also stores the whitened residuals W r
+ * (C x ldw FP32, ldw >= N rounded up to 128, C a multiple of 128): the
+ * training fields of the surrogate, produced by the same evaluation.
+ * Requires a lower-triangular whitener and a Mogi or CDM source. */
+int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
+                                 float *wr_dev, size_t ldw, void *stream);
+
 /* Model-specific constraints: mask_dev[c] += 1 for invalid samples (CDM: a
  * source above the free surface, where libcdm.py:219-224 raises).  No-op
  * for Mogi and linear. */
@@ -184,6 +191,62 @@ int altar_resample_gather(const double *theta_old, const double *prior_old, cons
                           double *prior, double *data, double *post, void *stream);
 /* post = prior + beta data (CoolingStep.computePosterior) */
 int altar_posterior(const double *prior_dev, const double *data_dev, double beta, int C, double *post_dev,
+                    void *stream);
+
+/* ---- dense algebra helpers (FP32 MFMA GEMM, row gather) */
+
+/* out = A B^T - bias (per column, optional): A M x K, B N x K, row-major;
+ * M, N multiples of 128, K a multiple of 16 */
+int altar_gemm_f32(const float *A_dev, size_t lda, const float *B_dev, size_t ldb, int M, int N, int K,
+                   const float *bias_dev, float *out_dev, size_t ldo, void *stream);
+/* dst[r, 0:cols] = src[idx[r], 0:cols], r < n */
+int altar_gather_rows_f32(const float *src_dev, size_t lds, const int *idx_dev, int n, int cols, float *dst_dev,
+                          size_t ldd, void *stream);
+
+/* ---- learned surrogate (no reference counterpart: the delayed-acceptance
+ * screen of the north-star design).  A dense network nin -> nhid x
+ * (nlayers - 1) -> nout with GELU hidden activations, BF16 weights with FP32
+ * master copies and Adam, FP32 accumulation on the matrix cores.  Inputs are
+ * parameter vectors scaled to [-1, 1] by altar_mlp_input_scaling. */
+typedef struct altar_mlp altar_mlp;
+int altar_mlp_create(altar_mlp **out, int nin, int nhid, int nlayers, int nout, int max_batch);
+int altar_mlp_destroy(altar_mlp *h);
+int altar_mlp_input_scaling(altar_mlp *h, const double *lo_host, const double *hi_host);
+/* layer l: W (out x in, row-major) and b (out), FP32 host; resets Adam state */
+int altar_mlp_set_layer(altar_mlp *h, int l, const float *W_host, const float *b_host);
+/* read layer l's master weights, or (grads != 0) the last gradients; synchronizes the device */
+int altar_mlp_get_layer(altar_mlp *h, int l, float *W_host, float *b_host, int grads);
+/* z (B x ldz FP32) = network(theta (B x nin FP64)) */
+int altar_mlp_forward(altar_mlp *h, const double *theta_dev, int B, float *z_dev, int ldz, void *stream);
+/* gradients of sum_b sum_k w_k (net_k - target_bk)^2 / B (target B x ldt FP32, w nout FP32) */
+int altar_mlp_backward(altar_mlp *h, const double *theta_dev, const float *target_dev, int ldt, const float *w_dev,
+                       int B, float *loss_dev, void *stream);
+/* one Adam step (Kingma and Ba 2015) with the last gradients */
+int altar_mlp_adam(altar_mlp *h, float lr, float beta1, float beta2, float eps, void *stream);
+/* surrogate data llk from normalized PCA coefficients z (C x ldz):
+ * llk = normalization - (cperp + sum_k (e_k + s_k z_k)^2) / 2; e, s device FP64 (K) */
+int altar_surrogate_llk(const float *z_dev, int ldz, int C, int K, const double *e_dev, const double *s_dev,
+                        double cperp, double normalization, double *llk_dev, void *stream);
+
+/* ---- two-stage delayed-acceptance Metropolis (Christen and Fox 2005) */
+/* stage 1: cpost_s = cprior + beta cdata_s; chain c passes if mask[c] == 0 and
+ * log u1 <= cpost_s - post_s (u1: screening stream).  index_dev (2 C ints)
+ * receives the passing chains in chain order, count_dev their number;
+ * counters_dev[1] += rejected, counters_dev[3] += screened out. */
+int altar_da_screen(const double *cprior_dev, const double *cdata_s_dev, const double *post_s_dev,
+                    const int *mask_dev, double beta, int C, uint64_t seed, uint32_t stage, uint32_t step,
+                    double *cpost_s_dev, int *index_dev, int *count_dev, unsigned long long *counters_dev,
+                    void *stream);
+/* packed[j] = cand[index[j]] for j < n (FP64 rows of P) */
+int altar_da_pack(const double *cand_dev, const int *index_dev, int n, int P, double *packed_dev, void *stream);
+/* stage 2 for the n screened-in chains with their exact data llk (packed):
+ * accept if log u2 <= (cpost - post) - (cpost_s - post_s); on acceptance the
+ * state, post and post_s take the candidate's values.  counters_dev[0] +=
+ * accepted, counters_dev[2] += unlikely at stage 2. */
+int altar_da_accept(double *theta_dev, double *prior_dev, double *data_dev, double *post_dev, double *post_s_dev,
+                    const double *cand_dev, const double *cprior_dev, const double *cdata_packed_dev,
+                    const double *cpost_s_dev, const int *index_dev, int n, double beta, int P, uint64_t seed,
+                    uint32_t stage, uint32_t step, int *decision_dev, unsigned long long *counters_dev,
                     void *stream);
 
 /* Philox normals and uniforms into device memory (the same streams the

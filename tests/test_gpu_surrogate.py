@@ -1,0 +1,132 @@
+"""Surrogate network kernels against the FP64 oracle (oracle/mlp.py).
+
+* forward: relative error of the network output <= 2e-2 (BF16 operands)
+* gradients: per layer, relative Frobenius error <= 5e-2 and cosine >= 0.995
+  against the chain-rule gradients of the same (BF16-rounded) weights
+* training: Adam on a smooth target reduces the loss by 10x
+* delayed acceptance: the screen list, the stage-2 rule and the counters
+  match a numpy restatement of Christen and Fox (2005) on the same uniforms
+"""
+import numpy as np
+import pytest
+import torch
+
+from altar_amd import _lib
+from altar_amd.surrogate import MLP
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    return torch.from_numpy(np.asarray(a, np.float32)).bfloat16().float().numpy().astype(np.float64)
+
+
+def setup_net(seed=0, nin=10, nhid=256, nout=64):
+    net = MLP(nin, nhid, 4, nout, max_batch=1024).init(seed)
+    lo, hi = -np.ones(nin) * 5, np.ones(nin) * 5
+    net.input_scaling(lo, hi)
+    layers = [tuple(a.astype(np.float64) for a in net.layer(l)) for l in range(4)]
+    return net, layers, lo, hi
+
+
+def test_mlp_forward_and_gradients(cuda):
+    from oracle import mlp as om
+    net, layers, lo, hi = setup_net()
+    rng = np.random.default_rng(0)
+    B = 200   # not a multiple of the 64-row tile
+    theta = rng.uniform(-5, 5, (B, 10))
+    target = rng.normal(size=(B, 64))
+    w = rng.random(64)
+    w /= w.sum()
+    t = torch.from_numpy(theta).to(cuda)
+    z = net.forward(t).cpu().numpy().astype(np.float64)
+    x = bf16(om.scale_inputs(theta, lo, hi))
+    ref_layers = [(bf16(W), b) for W, b in layers]
+    y, _ = om.forward(ref_layers, x)
+    assert np.linalg.norm(z - y) / np.linalg.norm(y) < 2e-2
+    tgt = torch.from_numpy(target.astype(np.float32)).to(cuda)
+    wd = torch.from_numpy(w.astype(np.float32)).to(cuda)
+    loss = torch.zeros(1, dtype=torch.float32, device=cuda)
+    net.backward(t, tgt, wd, loss)
+    ref_loss, grads = om.loss_and_grads(ref_layers, x, target, w)
+    assert abs(loss.item() - ref_loss) / ref_loss < 2e-2
+    for l in range(4):
+        gW, gb = (a.astype(np.float64) for a in net.layer(l, grads=True))
+        rW, rb = grads[l]
+        rel = np.linalg.norm(gW - rW) / np.linalg.norm(rW)
+        cos = np.sum(gW * rW) / (np.linalg.norm(gW) * np.linalg.norm(rW))
+        print(f"layer {l}: dW rel {rel:.3e} cos {cos:.5f}; db rel {np.linalg.norm(gb - rb) / np.linalg.norm(rb):.3e}")
+        assert rel < 5e-2 and cos > 0.995
+        assert np.linalg.norm(gb - rb) / np.linalg.norm(rb) < 5e-2
+
+
+def test_mlp_learns_smooth_map(cuda):
+    net, _, lo, hi = setup_net(1, nin=4, nout=64)
+    rng = np.random.default_rng(2)
+    B = 1024
+    A = rng.normal(size=(4, 64))
+
+    def f(th):
+        return np.tanh(th / 5.0 @ A)
+
+    w = torch.full((64,), 1.0 / 64, dtype=torch.float32, device=cuda)
+    loss = torch.zeros(1, dtype=torch.float32, device=cuda)
+    losses = []
+    for it in range(300):
+        th = rng.uniform(-5, 5, (B, 4))
+        t = torch.from_numpy(th).to(cuda)
+        tgt = torch.from_numpy(f(th).astype(np.float32)).to(cuda)
+        net.backward(t, tgt, w, loss)
+        net.adam(1e-3)
+        losses.append(loss.item())
+    print("loss first/last", losses[0], losses[-1])
+    assert losses[-1] < losses[0] / 10
+
+
+def test_delayed_acceptance_rule(cuda, oracle):
+    rng = np.random.default_rng(3)
+    C, P = 3000, 4
+    beta = 0.6
+    theta = rng.normal(size=(C, P))
+    prior, data = rng.normal(size=C), rng.normal(size=C) * 5
+    post = prior + beta * data
+    post_s = post + rng.normal(0, 0.5, C)
+    cand = rng.normal(size=(C, P))
+    cprior, cdata_s = rng.normal(size=C), rng.normal(size=C) * 5
+    cdata = cdata_s + rng.normal(0, 0.5, C)
+    mask = (rng.random(C) < 0.1).astype(np.int32)
+    seed, stage, step = 9, 4, 2
+    # numpy restatement
+    cpost_s = cprior + beta * cdata_s
+    u1 = oracle.uniforms(seed, stage, step, _lib.TAG_SCREEN, C)
+    passed = (mask == 0) & (np.log(u1) <= cpost_s - post_s)
+    idx = np.nonzero(passed)[0]
+    u2 = oracle.uniforms(seed, stage, step, _lib.TAG_ACCEPT, C)
+    cpost = cprior + beta * cdata
+    acc = np.zeros(C, bool)
+    acc[idx] = np.log(u2[idx]) <= (cpost[idx] - post[idx]) - (cpost_s[idx] - post_s[idx])
+    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    th_d, pr_d, da_d, po_d, ps_d = D(theta.copy()), D(prior.copy()), D(data.copy()), D(post.copy()), D(post_s.copy())
+    cpost_s_d = torch.empty(C, dtype=torch.float64, device=cuda)
+    index = torch.empty(2 * C, dtype=torch.int32, device=cuda)
+    count = torch.zeros(1, dtype=torch.int32, device=cuda)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    _lib.call("altar_da_screen", _lib.ptr(D(cprior)), _lib.ptr(D(cdata_s)), _lib.ptr(ps_d), _lib.ptr(D(mask)), beta,
+              C, seed, stage, step, _lib.ptr(cpost_s_d), _lib.ptr(index), _lib.ptr(count), _lib.ptr(counters), None)
+    n = int(count.item())
+    assert n == len(idx)
+    assert np.array_equal(index[:n].cpu().numpy(), idx)
+    packed = torch.empty((n, P), dtype=torch.float64, device=cuda)
+    cd = D(cand)
+    _lib.call("altar_da_pack", _lib.ptr(cd), _lib.ptr(index), n, P, _lib.ptr(packed), None)
+    assert np.array_equal(packed.cpu().numpy(), cand[idx])
+    _lib.call("altar_da_accept", _lib.ptr(th_d), _lib.ptr(pr_d), _lib.ptr(da_d), _lib.ptr(po_d), _lib.ptr(ps_d),
+              _lib.ptr(cd), _lib.ptr(D(cprior)), _lib.ptr(D(cdata[idx])), _lib.ptr(cpost_s_d), _lib.ptr(index), n,
+              beta, P, seed, stage, step, None, _lib.ptr(counters), None)
+    th = th_d.cpu().numpy()
+    assert np.array_equal(th[acc], cand[acc]) and np.array_equal(th[~acc], theta[~acc])
+    assert np.array_equal(po_d.cpu().numpy()[acc], cpost[acc])
+    assert np.array_equal(ps_d.cpu().numpy()[acc], cpost_s[acc])
+    cnt = counters.cpu().numpy()
+    assert cnt[0] == acc.sum() and cnt[1] == mask.sum()
+    assert cnt[3] == ((mask == 0) & ~passed).sum() and cnt[2] == len(idx) - acc.sum()
