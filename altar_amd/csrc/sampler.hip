@@ -12,6 +12,11 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// The state updates (posterior = prior + beta data, the accept test) follow
+// the reference's separate multiply and add; no fused multiply-add
+// contraction in this file.
+#pragma clang fp contract(off)
+
 namespace altar {
 
 // ---------------------------------------------------------------- draws

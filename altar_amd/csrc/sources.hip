@@ -159,10 +159,10 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                 dx[v] = X - G.vx[v];
                 dy[v] = Y - G.vy[v];
                 float a = G.va[v];
-                r[v] = sqrtf(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
-                ir[v] = 1.0f / r[v];
+                r[v] = cdm::f_sqrt(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
+                ir[v] = cdm::f_rcp(r[v]);
                 rpa[v] = r[v] + a;
-                irpa[v] = 1.0f / rpa[v];
+                irpa[v] = cdm::f_rcp(rpa[v]);
             }
             const float k = 1.0f - 2.0f * nu;
             float ue = 0.0f, un = 0.0f, uv = 0.0f;

@@ -46,7 +46,22 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+    calibrate the HBM traffic counters of the kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
 def cpu_baseline(case, theta, seconds):
@@ -73,6 +88,9 @@ def cpu_baseline(case, theta, seconds):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(chain-parallel sharding is described in DESIGN.md, not built)"}))
