@@ -84,26 +84,27 @@ ALTAR_HD inline float f_atan2(float y, float x)
 }
 ALTAR_HD inline double f_atan2(double y, double x) { return atan2(y, x); }
 
-// side constants of one angular-dislocation pair for one configuration
-// (configuration I uses the angle beta - pi, II uses beta; libcdm.py:344-362)
-template <typename T>
-struct SideCfg {
-    T s, c, t, h;        // sin, cos, cot of the angle, and cot of half the angle
-    T cF[3], cL[3], cZ[3];  // coefficients of dFi/2, dlog(r+a), dlog(r-z3) in v1..v3
-};
-
+// Side constants of one angular-dislocation pair.  Configuration II uses
+// the angle beta and configuration I the angle beta - pi (libcdm.py:344-362):
+// sin and cos change sign, cot does not, so the configuration-II values are
+// stored and configuration I flips the sign of s, c, cZ and e13.  cF, cL, cZ
+// multiply the transcendental terms dFi/2, dlog(r+a), dlog(r-z3) of v1..v3.
 template <typename T>
 struct Side {
     T e1x, e1y;          // first ADCS axis (horizontal); ey2 = (e1y, -e1x, 0), ey3 = -ez
     T y1AB, y2AB;        // ADCS coordinates of PB - PA (libcdm.py:337-339)
     T b1, b2, b3;        // Burgers vector in ADCS, divided by 2 pi
-    SideCfg<T> cfg[2];   // [0]: configuration I, [1]: configuration II
+    T s, c, t;           // sin, cos, cot of beta (configuration II)
+    T hI, hII;           // cot of half the angle, configurations I and II
+    T kt, e13;           // k cot(beta) and b3 sin(beta) - b1 cos(beta)
+    T cF[3], cL[3], cZ[3];
 };
 
 // per-chain geometry: 12 vertices (x, y, depth a = -z) and 12 sides
 template <typename T>
 struct Geom {
     T vx[12], vy[12], va[12];
+    T ka2[12];           // (1 - 2 nu) a / 2 per vertex
     Side<T> side[12];
 };
 
@@ -113,39 +114,46 @@ ALTAR_HD constexpr int side_vB(int s) { return (s / 4) * 4 + ((s % 4) + 1) % 4; 
 
 // Rational part of AngDisDispSurf at one vertex (libcdm.py:394-425 with the
 // Fi, log(r+a) and log(r-z3) terms removed), contracted with the Burgers
-// vector.  Also returns r - z3 and the two atan2 arguments of Fi.
+// vector.  With z1 = y1 c + a s, z3 = y1 s - a c, u = r s - y1,
+// q = 1/(r (r - z3)), H = k (t + y1 / (2 (r+a))) - y1 / r and
+// W1 = y2 (b3 s - b1 c) + b2 z1, the nine terms of the reference collapse to
+//   v1 = b1 y2 H/(r+a) - b2 (y1 H + k a/2)/(r+a) + u q W1
+//   v2 = -b1 (k y1 t - k a/2 + y2 (y2/r - k y2 / (2 (r+a))))/(r+a)
+//        - b2 y2 H/(r+a) - y2 q W1
+//   v3 = (2 nu + a/r)(b1 y2 - b2 y1)/(r+a) + (c + a/r)(W1 - b3 s y2)/(r - z3)
+//        + b3 s y2 q (r c + a)
+// (k = 1 - 2 nu; s, c the configuration's sine and cosine).  Also returns
+// r - z3 as the fraction p / qd (qd = 1 when z3 <= 0; for z3 > 0,
+// r - z3 = (z1^2 + y2^2) / (r + z3) without cancellation) and the atan2
+// arguments of Fi.
 template <typename T>
-ALTAR_HD inline void vertex_terms(T y1, T y2, T a, T r, T ir, T rpa, T irpa,
-                                  const SideCfg<T> &g, T nu, T k, T b1, T b2, T b3,
-                                  T &v1, T &v2, T &v3, T &rmz, T &fy, T &fx)
+ALTAR_HD inline void vertex_terms(T y1, T y2, T a, T ka2, T r, T ir, T irpa, T s, T c, T h, T e13,
+                                  const Side<T> &sd, T k05, T nu2, T &v1, T &v2, T &v3, T &p, T &qd,
+                                  T &fy, T &fx, T &rpa)
 {
-    const T s = g.s, c = g.c, t = g.t;
-    T z1 = y1 * c + a * s;
-    T z3 = y1 * s - a * c;
-    T rmz_ = z3 > T(0) ? f_div(z1 * z1 + y2 * y2, r + z3) : r - z3;
-    T irmz = f_rcp(rmz_);
-    T q = ir * irmz;
-    T u = r * s - y1;
-    T y1sq = y1 * y1, y2sq = y2 * y2;
-    T h = k * (t + T(0.5) * y1 * irpa) - y1 * ir;
-    T gg = T(2) * nu + a * ir;
-    T ca = c + a * ir;
-    T r1b1 = y2 * irpa * h - y2 * u * c * q;
-    T r2b1 = -irpa * (k * (y1 * t - T(0.5) * a - T(0.5) * y2sq * irpa) + y2sq * ir) + y2sq * c * q;
-    T r3b1 = y2 * irpa * gg - y2 * c * irmz * ca;
-    T r1b2 = -irpa * (k * (y1 * t + T(0.5) * a + T(0.5) * y1sq * irpa) - y1sq * ir) + z1 * u * q;
-    T r2b2 = -y2 * irpa * h - y2 * z1 * q;
-    T r3b2 = -y1 * irpa * gg + z1 * irmz * ca;
-    T sq = s * q;
-    T r1b3 = y2 * u * sq;
-    T r2b3 = -y2sq * sq;
-    T r3b3 = y2 * (r * c + a) * sq;
-    v1 = b1 * r1b1 + b2 * r1b2 + b3 * r1b3;
-    v2 = b1 * r2b1 + b2 * r2b2 + b3 * r2b3;
-    v3 = b1 * r3b1 + b2 * r3b2 + b3 * r3b3;
-    rmz = rmz_;
+    const T z1 = y1 * c + a * s;
+    const T z3 = y1 * s - a * c;
+    const bool pos = z3 > T(0);
+    p = pos ? z1 * z1 + y2 * y2 : r - z3;
+    qd = pos ? r + z3 : T(1);
+    const T irmz = qd * f_rcp(p);
+    const T q = ir * irmz;
+    const T u = r * s - y1;
+    const T Y2I = y2 * irpa, Y1I = y1 * irpa;
+    const T H = (sd.kt + k05 * Y1I) - y1 * ir;
+    const T W1 = y2 * e13 + sd.b2 * z1;
+    const T P1 = Y2I * H;
+    const T t1 = y1 * H + ka2;
+    v1 = sd.b1 * P1 - (sd.b2 * irpa) * t1 + (u * q) * W1;
+    const T inner2 = (sd.kt * y1 - ka2) + y2 * (y2 * ir - k05 * Y2I);
+    v2 = -(sd.b1 * irpa) * inner2 - sd.b2 * P1 - (y2 * q) * W1;
+    const T gg = nu2 + a * ir;
+    const T ca = c + a * ir;
+    const T G3 = (sd.b3 * s) * y2;
+    v3 = gg * (sd.b1 * Y2I - sd.b2 * Y1I) + (irmz * ca) * (W1 - G3) + (G3 * q) * (r * c + a);
+    rpa = r + a;
     fy = y2;
-    fx = rpa * g.h - y1;
+    fx = rpa * h - y1;
 }
 
 // Host-side (FP64) construction of the side constants, following
@@ -177,26 +185,26 @@ ALTAR_HD inline bool make_side(const double PA[3], const double PB[3], const dou
     sd.b2 = b2 * i2pi;
     sd.b3 = b3 * i2pi;
     const double k = 1.0 - 2.0 * nu;
-    for (int cfg = 0; cfg < 2; ++cfg) {
-        double ang = cfg == 0 ? -pi + beta : beta;
-        SideCfg<double> &g = sd.cfg[cfg];
-        g.s = sin(ang);
-        g.c = cos(ang);
-        g.t = 1.0 / tan(ang);
-        g.h = 1.0 / tan(ang / 2.0);
-        double s = g.s, c = g.c, t = g.t;
-        double B1 = sd.b1, B2 = sd.b2, B3 = sd.b3;
-        // Fi enters as 2 * atan2(...): the factor 2 is folded in here
-        g.cF[0] = 2.0 * (B1 * (1.0 - k * t * t));
-        g.cF[1] = 2.0 * (B2 * (1.0 + k * t * t));
-        g.cF[2] = 2.0 * (B1 * k * t + B3);
-        g.cL[0] = -B2 * k * (0.5 - t * t);
-        g.cL[1] = B1 * k * (0.5 + t * t);
-        g.cL[2] = -B2 * k * t;
-        g.cZ[0] = -B2 * k * t * t * c;
-        g.cZ[1] = -B1 * k * t / s;
-        g.cZ[2] = B2 * k * t * c;
-    }
+    // configuration II: the angle beta; configuration I: beta - pi
+    const double sB = sin(beta), cB = cos(beta), tB = 1.0 / tan(beta);
+    sd.s = sB;
+    sd.c = cB;
+    sd.t = tB;
+    sd.hII = 1.0 / tan(beta / 2.0);
+    sd.hI = 1.0 / tan((-pi + beta) / 2.0);
+    sd.kt = k * tB;
+    double B1 = sd.b1, B2 = sd.b2, B3 = sd.b3;
+    sd.e13 = B3 * sB - B1 * cB;
+    // Fi enters as 2 * atan2(...): the factor 2 is folded in here
+    sd.cF[0] = 2.0 * (B1 * (1.0 - k * tB * tB));
+    sd.cF[1] = 2.0 * (B2 * (1.0 + k * tB * tB));
+    sd.cF[2] = 2.0 * (B1 * k * tB + B3);
+    sd.cL[0] = -B2 * k * (0.5 - tB * tB);
+    sd.cL[1] = B1 * k * (0.5 + tB * tB);
+    sd.cL[2] = -B2 * k * tB;
+    sd.cZ[0] = -B2 * k * tB * tB * cB;
+    sd.cZ[1] = -B1 * k * tB / sB;
+    sd.cZ[2] = B2 * k * tB * cB;
     return true;
 }
 
@@ -266,6 +274,7 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
         G.vx[v] = V[v][0];
         G.vy[v] = V[v][1];
         G.va[v] = -V[v][2];
+        G.ka2[v] = 0.5 * (1.0 - 2.0 * nu) * G.va[v];
     }
     // which rectangles contribute (libcdm.py:226-243, on the full axes)
     double Ax = 2.0 * ax, Ay = 2.0 * ay, Az = 2.0 * az;
@@ -301,34 +310,42 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
 }
 
 // Displacement (ue, un, uv) of one side at a point, everything in type T.
-// dxA = X - xA etc.; rA, irA, rpaA, irpaA are the per-vertex quantities.
-// The second vertex's ADCS coordinates are formed from the first one's as
-// in libcdm.py:337-339, so both share the sign of y2 on the Burgers
-// function's branch cut.
+// dxA = X - xA etc.; rA, irA, irpaA are the per-vertex quantities (r, 1/r,
+// 1/(r+a)).  The second vertex's ADCS coordinates are formed from the first
+// one's as in libcdm.py:337-339, so both share the sign of y2 on the Burgers
+// function's branch cut.  With y2 of one sign at both vertices, both Burgers
+// angles lie in the same half plane and their difference is one atan2:
+// atan2(yB xA - yA xB, xA xB + yA yB).
 template <typename T>
 ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
-                               T dxA, T dyA, T aA, T rA, T irA, T rpaA, T irpaA,
-                               T aB, T rB, T irB, T rpaB, T irpaB,
+                               T dxA, T dyA, T aA, T kaA, T rA, T irA, T irpaA,
+                               T aB, T kaB, T rB, T irB, T irpaB,
                                T &ue, T &un, T &uv)
 {
-    T e1x = sd.e1x, e1y = sd.e1y;
-    T y1A = e1x * dxA + e1y * dyA;
-    T y2A = e1y * dxA - e1x * dyA;
-    T y1B = y1A - sd.y1AB;
-    T y2B = y2A - sd.y2AB;
-    const SideCfg<T> &g = y1A >= T(0) ? sd.cfg[0] : sd.cfg[1];
-    T v1A, v2A, v3A, rmzA, fyA, fxA;
-    T v1B, v2B, v3B, rmzB, fyB, fxB;
-    vertex_terms<T>(y1A, y2A, aA, rA, irA, rpaA, irpaA, g, nu, k, sd.b1, sd.b2, sd.b3,
-                    v1A, v2A, v3A, rmzA, fyA, fxA);
-    vertex_terms<T>(y1B, y2B, aB, rB, irB, rpaB, irpaB, g, nu, k, sd.b1, sd.b2, sd.b3,
-                    v1B, v2B, v3B, rmzB, fyB, fxB);
-    T dFi = f_atan2(fyB, fxB) - f_atan2(fyA, fxA);
-    T dLr = f_log(f_div(rpaB, rpaA));
-    T dLz = f_log(f_div(rmzB, rmzA));
-    T v1 = (v1B - v1A) + g.cF[0] * dFi + g.cL[0] * dLr + g.cZ[0] * dLz;
-    T v2 = (v2B - v2A) + g.cF[1] * dFi + g.cL[1] * dLr + g.cZ[1] * dLz;
-    T v3 = (v3B - v3A) + g.cF[2] * dFi + g.cL[2] * dLr + g.cZ[2] * dLz;
+    const T e1x = sd.e1x, e1y = sd.e1y;
+    const T y1A = e1x * dxA + e1y * dyA;
+    const T y2A = e1y * dxA - e1x * dyA;
+    const T y1B = y1A - sd.y1AB;
+    const T y2B = y2A - sd.y2AB;
+    // configuration I (y1A >= 0) uses beta - pi: sin, cos, cZ, e13 change sign
+    const bool cfgI = y1A >= T(0);
+    const T sg = cfgI ? T(-1) : T(1);
+    const T s = sg * sd.s, c = sg * sd.c, e13 = sg * sd.e13;
+    const T h = cfgI ? sd.hI : sd.hII;
+    const T k05 = T(0.5) * k, nu2 = T(2) * nu;
+    T v1A, v2A, v3A, pA, qA, fyA, fxA, rpaA;
+    T v1B, v2B, v3B, pB, qB, fyB, fxB, rpaB;
+    vertex_terms<T>(y1A, y2A, aA, kaA, rA, irA, irpaA, s, c, h, e13, sd, k05, nu2,
+                    v1A, v2A, v3A, pA, qA, fyA, fxA, rpaA);
+    vertex_terms<T>(y1B, y2B, aB, kaB, rB, irB, irpaB, s, c, h, e13, sd, k05, nu2,
+                    v1B, v2B, v3B, pB, qB, fyB, fxB, rpaB);
+    const T dFi = f_atan2(fyB * fxA - fyA * fxB, fxA * fxB + fyA * fyB);
+    // log((rB + aB) / (rA + aA)) and log((rB - z3B) / (rA - z3A))
+    const T dLr = f_log(rpaB * irpaA);
+    const T dLz = sg * f_log(f_div(pB * qA, qB * pA));
+    const T v1 = (v1B - v1A) + sd.cF[0] * dFi + sd.cL[0] * dLr + sd.cZ[0] * dLz;
+    const T v2 = (v2B - v2A) + sd.cF[1] * dFi + sd.cL[1] * dLr + sd.cZ[1] * dLz;
+    const T v3 = (v3B - v3A) + sd.cF[2] * dFi + sd.cL[2] * dLr + sd.cZ[2] * dLz;
     ue = e1x * v1 + e1y * v2;
     un = e1y * v1 - e1x * v2;
     uv = -v3;
@@ -341,6 +358,7 @@ ALTAR_HD inline void convert(const Geom<S> &in, Geom<T> &out)
         out.vx[v] = (T)in.vx[v];
         out.vy[v] = (T)in.vy[v];
         out.va[v] = (T)in.va[v];
+        out.ka2[v] = (T)in.ka2[v];
     }
     for (int s = 0; s < 12; ++s) {
         const Side<S> &a = in.side[s];
@@ -348,14 +366,13 @@ ALTAR_HD inline void convert(const Geom<S> &in, Geom<T> &out)
         b.e1x = (T)a.e1x; b.e1y = (T)a.e1y;
         b.y1AB = (T)a.y1AB; b.y2AB = (T)a.y2AB;
         b.b1 = (T)a.b1; b.b2 = (T)a.b2; b.b3 = (T)a.b3;
-        for (int c = 0; c < 2; ++c) {
-            b.cfg[c].s = (T)a.cfg[c].s; b.cfg[c].c = (T)a.cfg[c].c;
-            b.cfg[c].t = (T)a.cfg[c].t; b.cfg[c].h = (T)a.cfg[c].h;
-            for (int j = 0; j < 3; ++j) {
-                b.cfg[c].cF[j] = (T)a.cfg[c].cF[j];
-                b.cfg[c].cL[j] = (T)a.cfg[c].cL[j];
-                b.cfg[c].cZ[j] = (T)a.cfg[c].cZ[j];
-            }
+        b.s = (T)a.s; b.c = (T)a.c; b.t = (T)a.t;
+        b.hI = (T)a.hI; b.hII = (T)a.hII;
+        b.kt = (T)a.kt; b.e13 = (T)a.e13;
+        for (int j = 0; j < 3; ++j) {
+            b.cF[j] = (T)a.cF[j];
+            b.cL[j] = (T)a.cL[j];
+            b.cZ[j] = (T)a.cZ[j];
         }
     }
 }
@@ -366,15 +383,14 @@ template <typename T>
 inline void point_disp(const Geom<T> &G, const Geom<double> &G64, unsigned flags, double nu,
                        double X, double Y, double &ue, double &un, double &uv)
 {
-    T r[12], ir[12], rpa[12], irpa[12], dx[12], dy[12];
+    T r[12], ir[12], irpa[12], dx[12], dy[12];
     for (int v = 0; v < 12; ++v) {
         dx[v] = (T)X - G.vx[v];
         dy[v] = (T)Y - G.vy[v];
         T a = G.va[v];
         r[v] = sqrt(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
         ir[v] = T(1) / r[v];
-        rpa[v] = r[v] + a;
-        irpa[v] = T(1) / rpa[v];
+        irpa[v] = T(1) / (r[v] + a);
     }
     T sue = 0, sun = 0, suv = 0;
     double due = 0, dun = 0, duv = 0;
@@ -388,14 +404,13 @@ inline void point_disp(const Geom<T> &G, const Geom<double> &G64, unsigned flags
             double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
             double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
             double e, n, u;
-            side_disp<double>(G64.side[s], nu, 1.0 - 2.0 * nu, dxa, dya, aa, ra, 1.0 / ra, ra + aa,
-                              1.0 / (ra + aa), ab, rb, 1.0 / rb, rb + ab, 1.0 / (rb + ab),
-                              e, n, u);
+            side_disp<double>(G64.side[s], nu, 1.0 - 2.0 * nu, dxa, dya, aa, G64.ka2[a], ra, 1.0 / ra,
+                              1.0 / (ra + aa), ab, G64.ka2[b], rb, 1.0 / rb, 1.0 / (rb + ab), e, n, u);
             due += e; dun += n; duv += u;
         } else {
             T e, n, u;
-            side_disp<T>(G.side[s], tnu, tk, dx[a], dy[a], G.va[a], r[a], ir[a], rpa[a], irpa[a],
-                         G.va[b], r[b], ir[b], rpa[b], irpa[b], e, n, u);
+            side_disp<T>(G.side[s], tnu, tk, dx[a], dy[a], G.va[a], G.ka2[a], r[a], ir[a], irpa[a],
+                         G.va[b], G.ka2[b], r[b], ir[b], irpa[b], e, n, u);
             sue += e; sun += n; suv += u;
         }
     }

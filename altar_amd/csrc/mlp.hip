@@ -120,14 +120,25 @@ __global__ void mse_grad_kernel(const float *__restrict__ yhat, const float *__r
     if (loss && l != 0.0f) atomicAdd(loss, l);
 }
 
-// db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient), one thread per column
-__global__ void colsum_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld, float *__restrict__ db)
+// db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient).  A block owns 64
+// columns (one per lane, so every row read is coalesced); its 16 wavefronts
+// split the rows and their partial sums are added in a fixed order.
+__global__ __launch_bounds__(1024) void colsum_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld,
+                                                      float *__restrict__ db)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[16][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
     float s = 0.0f;
-    for (int r = 0; r < rows; ++r) s += bf2f(dy[(size_t)r * ld + c]);
-    db[c] = s;
+    if (c < cols)
+        for (int r = wid; r < rows; r += 16) s += bf2f(dy[(size_t)r * ld + c]);
+    part[wid][lane] = s;
+    __syncthreads();
+    if (wid == 0 && c < cols) {
+        float t = 0.0f;
+        for (int w = 0; w < 16; ++w) t += part[w][lane];
+        db[c] = t;
+    }
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy and
@@ -213,7 +224,7 @@ hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, in
 
 hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *db, hipStream_t st)
 {
-    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(64), 0, st, (const __bf16 *)dy, rows, cols, ld,
+    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(1024), 0, st, (const __bf16 *)dy, rows, cols, ld,
                        db);
     return hipGetLastError();
 }

@@ -136,7 +136,8 @@ __global__ void accept_kernel(double *__restrict__ theta, double *__restrict__ p
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int dec = -1;
     if (c < C) {
-        const double cp = __dadd_rn(cprior[c], __dmul_rn(beta, cdata[c]));
+        const double bd = beta * cdata[c];
+        const double cp = cprior[c] + bd;
         if (cpost_out) cpost_out[c] = cp;
         if (mask[c]) {
             dec = 1;
@@ -148,7 +149,7 @@ __global__ void accept_kernel(double *__restrict__ theta, double *__restrict__ p
                 u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_ACCEPT, 0u);
                 u = u53(r.x, r.y);
             }
-            const double diff = __dadd_rn(cp, -post[c]);
+            const double diff = cp - post[c];
             if (!(log(u) <= diff)) {
                 dec = 2;
             } else {
@@ -176,7 +177,10 @@ __global__ void posterior_kernel(const double *__restrict__ prior, const double 
                                  int C, double *__restrict__ post)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C) post[c] = __dadd_rn(prior[c], __dmul_rn(beta, data[c]));
+    if (c < C) {
+        const double bd = beta * data[c];
+        post[c] = prior[c] + bd;
+    }
 }
 
 // weighted mean of every parameter (COV.py:137-140): one thread per parameter
@@ -329,14 +333,15 @@ __global__ void da_screen_kernel(const double *__restrict__ cprior, const double
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int pass = 0, rej = 0, scr = 0;
     if (c < C) {
-        const double cp = __dadd_rn(cprior[c], __dmul_rn(beta, cdata_s[c]));
+        const double bd = beta * cdata_s[c];
+        const double cp = cprior[c] + bd;
         cpost_s[c] = cp;
         if (mask[c]) {
             rej = 1;
         } else {
             u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_SCREEN, 0u);
             const double u = u53(r.x, r.y);
-            if (log(u) <= __dadd_rn(cp, -post_s[c])) pass = 1;
+            if (log(u) <= cp - post_s[c]) pass = 1;
             else scr = 1;
         }
         flag[c] = pass;
@@ -409,10 +414,11 @@ __global__ void da_accept_kernel(double *__restrict__ theta, double *__restrict_
     int acc = 0, unl = 0;
     if (j < n) {
         const int c = index[j];
-        const double cp = __dadd_rn(cprior[c], __dmul_rn(beta, cdata_packed[j]));
+        const double bd = beta * cdata_packed[j];
+        const double cp = cprior[c] + bd;
         u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_ACCEPT, 0u);
         const double u = u53(r.x, r.y);
-        const double lhs = __dadd_rn(__dadd_rn(cp, -post[c]), -__dadd_rn(cpost_s[c], -post_s[c]));
+        const double lhs = (cp - post[c]) - (cpost_s[c] - post_s[c]);
         if (log(u) <= lhs) {
             acc = 1;
             for (int k = 0; k < P; ++k) theta[(size_t)c * P + k] = cand[(size_t)c * P + k];

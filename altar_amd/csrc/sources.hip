@@ -153,7 +153,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
             val = __builtin_nanf("");
         } else {
             const float X = obs.x[n], Y = obs.y[n];
-            float dx[12], dy[12], r[12], ir[12], rpa[12], irpa[12];
+            float dx[12], dy[12], r[12], ir[12], irpa[12];
 #pragma unroll
             for (int v = 0; v < 12; ++v) {
                 dx[v] = X - G.vx[v];
@@ -161,8 +161,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                 float a = G.va[v];
                 r[v] = cdm::f_sqrt(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
                 ir[v] = cdm::f_rcp(r[v]);
-                rpa[v] = r[v] + a;
-                irpa[v] = cdm::f_rcp(rpa[v]);
+                irpa[v] = cdm::f_rcp(r[v] + a);
             }
             const float k = 1.0f - 2.0f * nu;
             float ue = 0.0f, un = 0.0f, uv = 0.0f;
@@ -181,14 +180,13 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                     double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
                     double e, nn, u;
                     cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya,
-                                           aa, ra, 1.0 / ra, ra + aa, 1.0 / (ra + aa), ab, rb,
-                                           1.0 / rb, rb + ab, 1.0 / (rb + ab), e, nn, u);
+                                           aa, H.ka2[a], ra, 1.0 / ra, 1.0 / (ra + aa), ab, H.ka2[b], rb,
+                                           1.0 / rb, 1.0 / (rb + ab), e, nn, u);
                     due += e; dun += nn; duv += u;
                 } else {
                     float e, nn, u;
-                    cdm::side_disp<float>(G.side[s], nu, k, dx[a], dy[a], G.va[a], r[a], ir[a],
-                                          rpa[a], irpa[a], G.va[b], r[b], ir[b], rpa[b], irpa[b],
-                                          e, nn, u);
+                    cdm::side_disp<float>(G.side[s], nu, k, dx[a], dy[a], G.va[a], G.ka2[a], r[a], ir[a],
+                                          irpa[a], G.va[b], G.ka2[b], r[b], ir[b], irpa[b], e, nn, u);
                     ue += e; un += nn; uv += u;
                 }
             }

@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FP32 flops per (chain, observation) of the CDM forward + residual,
 # counted from altar_amd/csrc/cdm_math.hpp and sources.hip (fused multiply-add
 # = 2, divide/sqrt/log/atan2 = 1 each); see the census in DESIGN.md.
-CDM_FLOPS_PER_PAIR = 2983
+CDM_FLOPS_PER_PAIR = 2719
 
 
 def parse():
@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-da", action="store_true", help="skip the surrogate delayed-acceptance leg")
+    ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -184,8 +186,82 @@ def main():
     if not args.no_cpu:
         theta = step.theta.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(case, theta, args.cpu_seconds)
+    if not args.no_da:
+        out["surrogate_da"] = da_leg(args, case, model, acc / elapsed)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def da_leg(args, case, model, plain_acc_rate):
+    """Anneal with field collection until beta >= 0.1, fit the surrogate
+    (PCA of whitened residuals + BF16 MLP), then time delayed-acceptance steps
+    and plain steps from the same state at the next temperature."""
+    import torch
+    from altar_amd import Job
+    from altar_amd.bayesian import Annealer, COV, Metropolis
+    from altar_amd.bayesian.da import DelayedAcceptance
+    from altar_amd.surrogate import Surrogate
+
+    C = args.chains
+    job = Job(chains=C, steps=5, seed=777 + 3)
+    model.job = job
+    t0 = time.perf_counter()
+    sur = Surrogate(model, K=64, capacity=args.da_train, max_batch=C, seed=3)
+    da = DelayedAcceptance(sur, train_beta=0.1, max_rmse=2e-3, steps=5,
+                           fit_kwargs=dict(epochs=40, batch=4096, lr=2e-3))
+    ann = Annealer(da, COV(), seed=job.seed).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    t_fit = None
+    while w.beta + job.tolerance < 1:
+        w.cool(ann)
+        stats = w.walk(ann)
+        w.resample(ann, stats)
+        if sur.trained and t_fit is None:
+            t_fit = time.perf_counter() - t0
+        if sur.trained and w.beta >= 0.1:
+            break
+    setup = time.perf_counter() - t0
+    if not sur.trained:
+        return {"error": "surrogate not trained", "beta": w.beta}
+    # one more stage: temperature, covariance, resampling; then time both samplers
+    w.cool(ann)
+    step = w.step
+    da.prepareSamplingPDF(ann, step)
+    ok = da._validate(ann, step)
+    state = step.clone()
+    K = max(2, args.steps)
+    da.steps = K
+    torch.cuda.synchronize()
+    ev0 = da.exact_evals
+    t1 = time.perf_counter()
+    acc_da, rej_da, unl_da = da._walk_da(ann, step) if ok else (0, 0, 0)
+    torch.cuda.synchronize()
+    t_da = time.perf_counter() - t1
+    exact = da.exact_evals - ev0
+    # plain Metropolis from the same state and temperature
+    plain = Metropolis(steps=K)
+    plain.scaling = da.scaling
+    plain.sigma_chol = da.sigma_chol
+    ann.sampler = plain
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    acc_p, rej_p, unl_p = plain.walkChains(ann, state)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t2
+    ann.sampler = da
+    return {
+        "beta": step.beta, "steps": K, "validation_rmse_m": sur.validation_rmse, "surrogate_used": bool(ok),
+        "training_fields": sur.count, "fit_s": t_fit, "setup_s": setup,
+        "da": {"ms_per_step": t_da / K * 1e3, "accepted": acc_da, "exact_evals": exact,
+               "accepted_samples_per_s": acc_da / t_da if t_da else None,
+               "exact_evals_per_s": exact / t_da if t_da else None},
+        "plain": {"ms_per_step": t_plain / K * 1e3, "accepted": acc_p, "exact_evals": C * K,
+                  "accepted_samples_per_s": acc_p / t_plain},
+        "gain_accepted_per_s": (acc_da / t_da) / (acc_p / t_plain) if (t_da and acc_p) else None,
+        "mlp_flops_per_chain": 2 * (10 * 256 + 256 * 256 * 2 + 256 * 64),
+    }
 
 
 if __name__ == "__main__":

@@ -105,7 +105,13 @@ def test_delayed_acceptance_rule(cuda, oracle):
     cpost = cprior + beta * cdata
     acc = np.zeros(C, bool)
     acc[idx] = np.log(u2[idx]) <= (cpost[idx] - post[idx]) - (cpost_s[idx] - post_s[idx])
-    D = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    keep = []
+
+    def D(a):   # device copy kept alive until the kernels have run
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+        keep.append(t)
+        return t
+
     th_d, pr_d, da_d, po_d, ps_d = D(theta.copy()), D(prior.copy()), D(data.copy()), D(post.copy()), D(post_s.copy())
     cpost_s_d = torch.empty(C, dtype=torch.float64, device=cuda)
     index = torch.empty(2 * C, dtype=torch.int32, device=cuda)
