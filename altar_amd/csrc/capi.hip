@@ -526,8 +526,8 @@ int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C
 {
     return guarded([&] {
         require(wr_dev != nullptr && ldw >= h->Npad, "fields: output row pitch must be >= the padded N");
-        require(h->wkind == ALTAR_WHITEN_LOWER && h->model != ALTAR_LINEAR,
-                "fields: a full whitener and a Mogi or CDM source are required");
+        require(h->wkind == ALTAR_WHITEN_LOWER && !(h->model == ALTAR_LINEAR && h->prewhitened),
+                "fields: a lower-triangular whitener is required (linear sources in direct mode)");
         require(precision == ALTAR_FP32 || precision == ALTAR_BF16 || precision == ALTAR_BF16_REFINED,
                 "fields: precision must be FP32 or BF16");
         data_llk_impl(h, theta_dev, C, llk_dev, precision, wr_dev, ldw, S(stream));

@@ -122,8 +122,12 @@ class Surrogate:
         self.count = 0
         self.mlp = MLP(self.P, hidden, layers, K, max_batch).init(seed)
         from .distributions import flatten
-        kinds, lo, hi = flatten(model.psets, "prior")
-        self.lo, self.hi = np.array(lo), np.array(hi)
+        kinds, a, b = flatten(model.psets, "prior")
+        kinds, a, b = np.array(kinds), np.array(a), np.array(b)
+        # network inputs scaled to [-1, 1] over a uniform support, or over
+        # mean +- 3 sigma of a gaussian prior
+        self.lo = np.where(kinds == 0, a, a - 3.0 * b)
+        self.hi = np.where(kinds == 0, b, a + 3.0 * b)
         self.mlp.input_scaling(self.lo, self.hi)
         self.trained = False
         self.validation_rmse = None

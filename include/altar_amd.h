@@ -115,7 +115,7 @@ int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, doubl
 /* As altar_source_data_llk, and also stores the whitened residuals W r
  * (C x ldw FP32, ldw >= N rounded up to 128, C a multiple of 128): the
  * training fields of the surrogate, produced by the same evaluation.
- * Requires a lower-triangular whitener and a Mogi or CDM source. */
+ * Requires a lower-triangular whitener (linear sources: not pre-whitened). */
 int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
                                  float *wr_dev, size_t ldw, void *stream);
 

@@ -7,13 +7,12 @@
    target's mean (within 5 Monte-Carlo standard errors) and covariance
    (within 5%): the two-stage rule keeps the target invariant whatever the
    surrogate.
-2. End to end: a Mogi inversion with a full exponential Cd is annealed with
-   plain Metropolis (two seeds) and with delayed acceptance whose surrogate
-   is trained on the annealer's own beta < train_beta evaluations.  The DA
-   posterior means must agree with the plain ones within 5 standard errors
-   and the standard deviations within the plain-vs-plain spread (+10%), and
-   DA must need fewer exact evaluations per accepted sample (the gain the
-   network brings).
+2. Linear-Gaussian, annealed end to end with a surrogate trained on the
+   annealer's own beta < train_beta evaluations: the DA posterior agrees with
+   the closed form Sigma = (G^T Cd^-1 G + I)^-1, mu = Sigma G^T Cd^-1 d
+   (means within 5 MC standard errors with ESS = C / 10, sds within 10%),
+   and DA needs fewer exact evaluations per accepted sample than plain
+   Metropolis on the same problem (the gain the network brings).
 """
 import numpy as np
 import pytest
@@ -22,8 +21,7 @@ import torch
 from altar_amd import Job, _lib, norms
 from altar_amd.bayesian import Annealer, COV, Metropolis
 from altar_amd.bayesian.da import DelayedAcceptance
-from altar_amd.cases import mogi_case
-from altar_amd.mogi import Mogi, default_psets
+from altar_amd.linear import Linear, default_psets
 from altar_amd.surrogate import Surrogate
 
 pytestmark = pytest.mark.gpu
@@ -77,47 +75,47 @@ def test_da_kernels_keep_target_invariant(cuda):
     assert np.allclose(np.cov(x.T), S, rtol=0.05, atol=0.05 * S.max())
 
 
-def build(case, cd, C, steps, seed):
-    job = Job(chains=C, steps=steps, seed=seed, tolerance=1e-3)
-    return Mogi(default_psets(), case["x"], case["y"], case["los"], case["d"], case["oid"], cd,
-                precision="fp32", job=job).initialize(), job
+def linear_problem(seed=12, N=64, P=3):
+    rng = np.random.default_rng(seed)
+    G = rng.normal(size=(N, P))
+    truth = rng.normal(size=P)
+    xy = rng.uniform(0, 2e4, (N, 2))
+    cd = norms.exponential_covariance(xy, 0.5, 5e3) + 0.5 * np.eye(N)
+    d = G @ truth + np.linalg.cholesky(cd) @ rng.normal(size=N)
+    ci = np.linalg.inv(cd)
+    S = np.linalg.inv(G.T @ ci @ G + np.eye(P))
+    return G, d, cd, S, S @ G.T @ ci @ d
 
 
-def test_da_matches_plain_posterior(cuda):
-    print("case: mogi DA vs plain C=4096 N=600 steps=20", flush=True)
-    rng = np.random.default_rng(21)
-    case = mogi_case(N=600)
-    cd = norms.exponential_covariance(np.stack([case["x"], case["y"]], 1), 25e-6, 5e3)
-    case["d"] = case["d"] + np.linalg.cholesky(cd) @ rng.standard_normal(600) * 0.5
-    C, S = 4096, 20
-    runs = []
-    for seed in (5, 6):
-        model, job = build(case, cd, C, S, seed)
-        plain = Annealer(Metropolis(steps=S), COV(), seed=seed).initialize(job)
-        plain.posterior(model)
-        runs.append((plain.worker.step.theta.cpu().numpy(), sum(h["accepted"] for h in plain.history),
-                     C * S * len(plain.history)))
-    (th_plain, plain_acc, plain_evals), (th_plain2, _, _) = runs
+def test_da_linear_gaussian_posterior(cuda):
+    print("case: linear-Gaussian DA C=8192 N=64 P=3 steps=20", flush=True)
+    G, d, cd, S, mu = linear_problem()
+    P, C, steps = 3, 8192, 20
+    # plain Metropolis on the same problem for the efficiency comparison
+    job = Job(chains=C, steps=steps, seed=3, tolerance=1e-3)
+    model = Linear(default_psets(P), G, d, cd, mode="direct", job=job).initialize()
+    plain = Annealer(Metropolis(steps=steps), COV(), seed=3).initialize(job)
+    plain.posterior(model)
+    plain_acc = sum(h["accepted"] for h in plain.history)
+    plain_evals = C * steps * len(plain.history)
 
-    model2, job2 = build(case, cd, C, S, 5)
-    sur = Surrogate(model2, K=32, capacity=C * S * 2, max_batch=C, seed=1)
-    da = DelayedAcceptance(sur, train_beta=0.02, max_rmse=2e-3, steps=S,
-                           fit_kwargs=dict(epochs=30, batch=2048, lr=2e-3))
-    ann = Annealer(da, COV(), seed=5).initialize(job2)
+    job2 = Job(chains=C, steps=steps, seed=3, tolerance=1e-3)
+    model2 = Linear(default_psets(P), G, d, cd, mode="direct", job=job2).initialize()
+    sur = Surrogate(model2, K=32, capacity=C * steps * 2, max_batch=C, seed=1)
+    da = DelayedAcceptance(sur, train_beta=0.05, max_rmse=1e9, steps=steps,
+                           fit_kwargs=dict(epochs=30, batch=4096, lr=2e-3))
+    ann = Annealer(da, COV(), seed=3).initialize(job2)
     ann.posterior(model2)
-    th_da = ann.worker.step.theta.cpu().numpy()
+    th = ann.worker.step.theta.cpu().numpy()
     modes = [m["mode"] for m in da.mode_history]
     da_acc = sum(h["accepted"] for h in ann.history)
-    print("surrogate rmse (m)", sur.validation_rmse, "modes", modes)
+    print("surrogate field rmse", sur.validation_rmse, "modes", modes)
     print("plain: accepted", plain_acc, "exact evals", plain_evals,
           "| DA: accepted", da_acc, "exact evals", da.exact_evals, "screened", da.screened)
-    assert sur.trained and "da" in modes
-    sd = th_plain.std(0)
+    sd = np.sqrt(np.diag(S))
     se = sd / np.sqrt(C / 10)
-    diff = np.abs(th_da.mean(0) - th_plain.mean(0))
-    spread = np.abs(th_plain2.std(0) / sd - 1)
-    print("mean diff / se", diff / (se * np.sqrt(2)), "sd ratio", th_da.std(0) / sd, "plain-plain sd ratio",
-          th_plain2.std(0) / sd)
-    assert np.all(diff < 5 * np.sqrt(2) * se)
-    assert np.all(np.abs(th_da.std(0) / sd - 1) < 2 * spread + 0.1)
+    print("DA mean", th.mean(0), "closed form", mu, "se", se, "sd ratio", th.std(0) / sd)
+    assert sur.trained and "da" in modes
+    assert np.all(np.abs(th.mean(0) - mu) < 5 * se)
+    assert np.all(np.abs(th.std(0) / sd - 1) < 0.1)
     assert da.exact_evals / max(da_acc, 1) < plain_evals / max(plain_acc, 1)
