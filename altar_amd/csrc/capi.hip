@@ -425,7 +425,9 @@ __global__ void split_kernel(const float *__restrict__ r, size_t n, __hip_bfloat
 
 void linear_residual_f32(altar_source *h, const double *theta, int C, bool fwd, hipStream_t st)
 {
-    const size_t Pp = h->Ppad, Np = h->Npad, Cp = h->Cpad;
+    // buffers are sized for the handle's capacity; the launches cover the
+    // rows of this call only (C rounded up to the 128-row tile)
+    const size_t Pp = h->Ppad, Np = h->Npad, Cp = h->Cpad, Mp = round_up((size_t)C, 128);
     h->T32.reserve(Cp * Pp * sizeof(float), true);
     h->Thi.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
     h->Tlo.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
@@ -438,7 +440,7 @@ void linear_residual_f32(altar_source *h, const double *theta, int C, bool fwd, 
     ep.out = h->R32.as<float>();
     ep.ldo = Np;
     Mark m(h, 0, st);
-    hip_check(launch_gemm_f32(h->T32.as<float>(), Pp, h->G32.as<float>(), Pp, (int)Cp, (int)Np, (int)Pp, 0,
+    hip_check(launch_gemm_f32(h->T32.as<float>(), Pp, h->G32.as<float>(), Pp, (int)Mp, (int)Np, (int)Pp, 0,
                               EPI_STORE, ep, st),
               "linear gemm");
 }
@@ -546,7 +548,10 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         require(precision >= ALTAR_FP64 && precision <= ALTAR_BF16_REFINED, "unknown precision");
         hipStream_t st = stream;
         h->ensure(C);
-        const size_t Np = h->Npad, Cp = h->Cpad;
+        // buffers are sized for the handle's capacity (Cp rows); every launch
+        // covers only the rows of this call, C rounded up to the 128-row tile
+        // (an output buffer of the caller has exactly those rows)
+        const size_t Np = h->Npad, Cp = h->Cpad, Mp = round_up((size_t)C, 128);
         const int ntn = (int)(Np / 128);
 
         if (h->model == ALTAR_LINEAR) {
@@ -568,11 +573,11 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
                 ep.ntiles = ntn;
                 Mark m(h, 1, st);
                 if (precision == ALTAR_FP32)
-                    hip_check(launch_gemm_f32(h->T32.as<float>(), Pp, h->G32.as<float>(), Pp, (int)Cp, (int)Np,
+                    hip_check(launch_gemm_f32(h->T32.as<float>(), Pp, h->G32.as<float>(), Pp, (int)Mp, (int)Np,
                                               (int)Pp, 0, EPI_SUMSQ_BIAS, ep, st),
                               "linear gemm");
                 else
-                    hip_check(launch_gemm_bf16x3(h->Thi.p, h->Tlo.p, Pp, h->Ghi.p, h->Glo.p, Pp, (int)Cp, (int)Np,
+                    hip_check(launch_gemm_bf16x3(h->Thi.p, h->Tlo.p, Pp, h->Ghi.p, h->Glo.p, Pp, (int)Mp, (int)Np,
                                                  (int)Pp, 0, precision == ALTAR_BF16, EPI_SUMSQ_BIAS, ep, st),
                               "linear gemm");
                 hip_check(launch_llk_reduce(h->partial.p, 0, ntn, C, h->normalization, llk_dev, st), "reduce");
@@ -630,14 +635,14 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
                 run_source(h, theta_dev, C, MODE_RES_F32, false, false, o, st);
             }
             Mark m(h, 1, st);
-            hip_check(launch_gemm_f32(h->R32.as<float>(), Np, h->W32.as<float>(), Np, (int)Cp, (int)Np, (int)Np, 1,
+            hip_check(launch_gemm_f32(h->R32.as<float>(), Np, h->W32.as<float>(), Np, (int)Mp, (int)Np, (int)Np, 1,
                                       epi, ep, st),
                       "whitening gemm");
         } else {
             h->Rhi.reserve(Cp * Np * sizeof(__hip_bfloat16), true);
             h->Rlo.reserve(Cp * Np * sizeof(__hip_bfloat16), true);
             if (h->model == ALTAR_LINEAR) {
-                size_t n = Cp * Np;
+                size_t n = Mp * Np;
                 hipLaunchKernelGGL(split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                                    h->R32.as<float>(), n, h->Rhi.as<__hip_bfloat16>(), h->Rlo.as<__hip_bfloat16>());
                 hip_check(hipGetLastError(), "split");
@@ -649,7 +654,7 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
                 run_source(h, theta_dev, C, MODE_RES_BF16, false, false, o, st);
             }
             Mark m(h, 1, st);
-            hip_check(launch_gemm_bf16x3(h->Rhi.p, h->Rlo.p, Np, h->Whi.p, h->Wlo.p, Np, (int)Cp, (int)Np, (int)Np, 1,
+            hip_check(launch_gemm_bf16x3(h->Rhi.p, h->Rlo.p, Np, h->Whi.p, h->Wlo.p, Np, (int)Mp, (int)Np, (int)Np, 1,
                                          precision == ALTAR_BF16, epi, ep, st),
                       "whitening gemm");
         }

@@ -96,7 +96,8 @@ def whitened_pca(wr, K, oversample=32, iterations=3, seed=0):
     mean = wr.mean(axis=0)
     X = wr - mean
     rng = np.random.default_rng(seed)
-    Q = np.linalg.qr(X @ rng.standard_normal((X.shape[1], K + oversample)))[0]
+    q = min(K + oversample, min(X.shape))
+    Q = np.linalg.qr(X @ rng.standard_normal((X.shape[1], q)))[0]
     for _ in range(iterations):
         Q = np.linalg.qr(X.T @ Q)[0]
         Q = np.linalg.qr(X @ Q)[0]

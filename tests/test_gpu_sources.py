@@ -211,3 +211,35 @@ def test_cdm_llk_full_cd(cuda, oracle):
         print(prec, "max rel llk err", rel.max(), "near-truth abs err", absn.max())
         if tol is not None:
             assert rel.max() < tol, prec
+
+
+def test_fields_write_only_the_callers_rows(cuda, oracle):
+    """altar_source_data_llk_fields after a larger call on the same handle:
+    the whitened residuals fill exactly C rows (rows past C keep a sentinel)
+    and equal W r of the oracle; the llk matches the plain entry point."""
+    from altar_amd import _lib as L
+    rng = np.random.default_rng(44)
+    case = mogi_case(N=300)
+    cd = norms.exponential_covariance(np.stack([case["x"], case["y"]], 1), 25e-6, 5e3)
+    src = make_mogi(case, 5)
+    kind, W = norms.whitener(cd)
+    norm = norms.normalization(cd)
+    src.whitening(kind, W, norm)
+    big = torch.from_numpy(mogi_theta(rng, 1024)).to(cuda)
+    out = torch.empty(1024, dtype=torch.float64, device=cuda)
+    src.data_llk(big, out, "fp32")                 # capacity 1024 chains
+    C, Npad, guard = 256, 384, 512
+    theta = mogi_theta(rng, C)
+    t = torch.from_numpy(theta).to(cuda)
+    buf = torch.full((C + guard, Npad), 12345.0, dtype=torch.float32, device=cuda)
+    llk = torch.empty(C, dtype=torch.float64, device=cuda)
+    L.call("altar_source_data_llk_fields", src.handle, L.ptr(t), C, L.ptr(llk), L.FP32, L.ptr(buf), Npad, None)
+    b = buf.cpu().numpy()
+    assert np.all(b[C:] == 12345.0)
+    res = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"], case["oid"], 0.25)
+    wr = res @ oracle.exact_whitener(cd).T
+    assert np.max(np.abs(b[:C, :300] - wr)) < 1e-4 * np.max(np.abs(wr))
+    assert np.all(b[:C, 300:] == 0.0)
+    ref = torch.empty(C, dtype=torch.float64, device=cuda)
+    src.data_llk(t, ref, "fp32")
+    assert np.allclose(llk.cpu().numpy(), ref.cpu().numpy(), rtol=1e-12, atol=0)
