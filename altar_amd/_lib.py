@@ -66,6 +66,8 @@ SIGNATURES = {
     "altar_cov_dbeta_grid": (_i, [_vp, _i, _d, _d, _d, _d, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "altar_cov_covariance": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "altar_cov_multiplicities": (_i, [_vp, _i, _u64, _u32, _vp]),
+    "altar_cov_update_dev": (_i, [_vp, _i, _d, _d, _d, _vp, _vp, _vp]),
+    "altar_cov_resample_dev": (_i, [_vp, _i, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
     "altar_resample_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "altar_posterior": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "altar_random_normals": (_i, [_vp, _i, _i, _u64, _u32, _u32, _u32, _vp]),

@@ -1271,3 +1271,33 @@ int altar_da_accept(double *theta_dev, double *prior_dev, double *data_dev, doub
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// GPU-resident COV update: median, dbeta grid search and weights on the
+// device (out_dev[0] = new beta, out_dev[1] = COV, out_dev[2] = median)
+int altar_cov_update_dev(const double *llk_dev, int C, double beta, double tolerance, double target, double *w_dev,
+                         double *out_dev, void *stream)
+{
+    return guarded([&] {
+        require(llk_dev && w_dev && out_dev && C > 1, "bad argument");
+        require(beta >= 0.0 && beta <= 1.0, "beta outside [0, 1]");
+        hip_check(launch_cov_update(llk_dev, C, beta, tolerance, target, w_dev, out_dev, S(stream)), "cov update");
+    });
+}
+
+// multinomial resampling on the device: src_dev[c] = old row of new row c
+// (original-sample order, each repeated by its multiplicity); count_dev (C)
+// receives the multiplicities; edges_dev (C + 1 doubles) and offset_dev (C
+// ints) are scratch
+int altar_cov_resample_dev(const double *w_dev, int C, uint64_t seed, uint32_t stage, double *edges_dev,
+                           int *count_dev, int *offset_dev, int *src_dev, void *stream)
+{
+    return guarded([&] {
+        require(w_dev && edges_dev && count_dev && offset_dev && src_dev && C > 0, "bad argument");
+        hip_check(launch_cov_resample_src(w_dev, C, seed, stage, edges_dev, count_dev, offset_dev, src_dev, S(stream)),
+                  "resample");
+    });
+}
+
+}  // extern "C"

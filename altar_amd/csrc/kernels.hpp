@@ -158,6 +158,12 @@ hipError_t launch_da_accept(double *theta, double *prior, double *data, double *
                             uint32_t stage, uint32_t step, int *decision, unsigned long long *counters,
                             hipStream_t st);
 
+// cov.hip
+hipError_t launch_cov_update(const double *llk, int C, double beta, double tolerance, double target, double *w,
+                             double *out, hipStream_t st);
+hipError_t launch_cov_resample_src(const double *w, int C, uint64_t seed, uint32_t stage, double *edges, int *count,
+                                   int *offset, int *src, hipStream_t st);
+
 hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int *mask, hipStream_t st);
 
 }  // namespace altar

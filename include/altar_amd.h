@@ -185,6 +185,17 @@ int altar_cov_covariance(const double *theta_dev, const double *w_dev, int C, in
  * stream (seed, stage) binned on the edges {0, cumsum(w)} (last edge forced
  * to 1); count_host (C).  Host, synchronous. */
 int altar_cov_multiplicities(const double *w_host, int C, uint64_t seed, uint32_t stage, int *count_host);
+/* The same COV update with the data llk resident on the device: median,
+ * grid search and weights in one workgroup (fixed-order FP64 reductions);
+ * out_dev[0] = new beta, out_dev[1] = COV, out_dev[2] = median. */
+int altar_cov_update_dev(const double *llk_dev, int C, double beta, double tolerance, double target, double *w_dev,
+                         double *out_dev, void *stream);
+/* multinomial resampling on the device with the same Philox stream as
+ * altar_cov_multiplicities: count_dev (C) the multiplicities, src_dev (C) the
+ * old row of each new row (original-sample order); edges_dev (C + 1
+ * doubles) and offset_dev (C ints) are scratch */
+int altar_cov_resample_dev(const double *w_dev, int C, uint64_t seed, uint32_t stage, double *edges_dev,
+                           int *count_dev, int *offset_dev, int *src_dev, void *stream);
 /* resampled state: row c <- old row src_dev[c] (theta, prior, data, posterior) */
 int altar_resample_gather(const double *theta_old, const double *prior_old, const double *data_old,
                           const double *post_old, const int *src_dev, int C, int P, double *theta,

@@ -131,3 +131,32 @@ def test_covariance_and_gather(cuda, oracle):
     assert np.array_equal(outs[0].cpu().numpy(), theta[src])
     for v, o in zip(vecs, outs[1:]):
         assert np.array_equal(o.cpu().numpy(), v.cpu().numpy()[src])
+
+
+@pytest.mark.parametrize("C", [16384, 4096, 1001])
+def test_resident_cov_matches_oracle(cuda, oracle, C):
+    """GPU-resident COV stage against the oracle on identical llk: median
+    exact, beta and COV to 1e-12, weights to 1e-12, multiplicities and the
+    resampling map identical on the same uniforms."""
+    rng = np.random.default_rng(C)
+    for beta0, scale in ((0.0, 30.0), (0.2, 3.0), (0.7, 0.3)):
+        llk = rng.normal(-800, scale, C)
+        d = torch.from_numpy(llk).to(cuda)
+        w = torch.empty(C, dtype=torch.float64, device=cuda)
+        out = torch.zeros(3, dtype=torch.float64, device=cuda)
+        _lib.call("altar_cov_update_dev", _lib.ptr(d), C, beta0, 0.01, 1.0, _lib.ptr(w), _lib.ptr(out), None)
+        beta, cov, med = out.cpu().numpy()
+        rb, rc, rw = oracle.dbeta_grid(llk, beta0)
+        assert med == np.median(llk)
+        assert abs(beta - rb) <= 1e-12 * max(1.0, rb) and abs(cov - rc) <= 1e-12 * rc
+        wg = w.cpu().numpy()
+        assert np.allclose(wg, rw, rtol=1e-12, atol=1e-300)
+        edges = torch.empty(C + 1, dtype=torch.float64, device=cuda)
+        count = torch.empty(C, dtype=torch.int32, device=cuda)
+        offset = torch.empty(C, dtype=torch.int32, device=cuda)
+        src = torch.empty(C, dtype=torch.int32, device=cuda)
+        _lib.call("altar_cov_resample_dev", _lib.ptr(w), C, 77, 5, _lib.ptr(edges), _lib.ptr(count), _lib.ptr(offset),
+                  _lib.ptr(src), None)
+        want = oracle.multiplicities(wg, oracle.uniforms(77, 5, 0, _lib.TAG_RESAMPLE, C))
+        assert np.array_equal(count.cpu().numpy(), want)
+        assert np.array_equal(src.cpu().numpy(), np.repeat(np.arange(C), want))
