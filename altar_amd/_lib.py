@@ -51,6 +51,8 @@ SIGNATURES = {
     "altar_source_data_llk": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "altar_source_verify": (_i, [_vp, _vp, _i, _vp, _vp]),
     "altar_source_data_llk_fields": (_i, [_vp, _vp, _i, _vp, _i, _vp, _sz, _vp]),
+    "altar_source_fp64_sides": (_i, [_vp, _d]),
+    "altar_source_side_census": (_i, [_vp, _i, _vp, _vp]),
     "altar_source_timing": (_i, [_vp, _i]),
     "altar_source_timers": (_i, [_vp, _vp, _vp, _i]),
     "altar_prior_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp]),

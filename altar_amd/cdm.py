@@ -17,7 +17,7 @@ from .source import HIP, Source, set_covariance
 
 class CDM(Bayesian):
     def __init__(self, psets, x, y, los, d, oid, cd, nu=0.25, whiten="exact", precision="fp32",
-                 job=None, device="cuda"):
+                 job=None, device="cuda", fp64_sides=0.4):
         super().__init__(psets, job=job, device=device)
         self.x, self.y = np.asarray(x, float), np.asarray(y, float)
         self.los, self.d = np.asarray(los, float), np.asarray(d, float)
@@ -26,6 +26,8 @@ class CDM(Bayesian):
         self.nu = nu
         self.whiten = whiten
         self.precision = precision
+        # sides with |sin(beta)| below this are evaluated in FP64
+        self.fp64_sides = fp64_sides
         self.observations = len(self.x)
 
     def initialize(self, application=None):
@@ -45,6 +47,7 @@ class CDM(Bayesian):
         self.source.layout([self.xIdx, self.dIdx, self.openingIdx, self.aXIdx, self.omegaXIdx,
                             self.offsetIdx, nsets])
         self.source.observations(self.x, self.y, self.los, self.d, self.oid)
+        self.source.fp64_sides(self.fp64_sides)
         if self.cd is not None:
             self.whitening_kind, self.W, self.normalization = set_covariance(self.source, self.cd, self.whiten)
         self.strategy = HIP().initialize(application=application, model=self)

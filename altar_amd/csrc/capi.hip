@@ -103,6 +103,7 @@ using namespace altar;
 struct altar_source {
     int model = 0, P = 0, N = 0;
     double nu = 0.25;
+    double fp64_sin = 0.4;  // CDM sides with |sin beta| below this run in FP64
     int layout[8] = {0};
     int nsets = 0;
     bool have_layout = false, have_obs = false, have_green = false, prewhitened = false;
@@ -118,7 +119,7 @@ struct altar_source {
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
-    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts;
+    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census;
     // optional per-kernel timing with HIP events on the launch stream
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[ALTAR_TIMER_SLOTS];
@@ -146,7 +147,7 @@ struct altar_source {
         Cpad = round_up((size_t)C, 128);
         // zero-initialised: the GEMMs read the padding columns
         R32.release(); Rhi.release(); Rlo.release(); R64.release(); T32.release(); Thi.release(); Tlo.release();
-        partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release();
+        partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release(); census.release();
     }
     size_t ntiles_obs() const { return (N + 255) / 256; }
 };
@@ -363,7 +364,7 @@ CdmArgs cdm_args(altar_source *h, const double *theta, int C, bool fwd)
     a.L = CdmLayout{h->layout[0], h->layout[1], h->layout[2], h->layout[3], h->layout[4], h->layout[5],
                     h->nsets};
     a.nu = h->nu;
-    a.fp64_sin = 0.4;
+    a.fp64_sin = h->fp64_sin;
     h->g32.reserve((size_t)h->Ccap * sizeof(cdm::Geom<float>));
     h->g64.reserve((size_t)h->Ccap * sizeof(cdm::Geom<double>));
     h->flags.reserve((size_t)h->Ccap * sizeof(unsigned));
@@ -937,6 +938,33 @@ int altar_random_normals(double *out_dev, int C, int P, uint64_t seed, uint32_t 
 }  // extern "C"
 
 extern "C" {
+
+int altar_source_fp64_sides(altar_source *h, double threshold)
+{
+    return guarded([&] {
+        require(h != nullptr, "null handle");
+        require(h->model == ALTAR_CDM, "fp64 sides: a CDM source is required");
+        require(threshold >= 0.0 && threshold <= 2.0, "fp64 sides: threshold must lie in [0, 2]");
+        h->fp64_sin = threshold;
+    });
+}
+
+int altar_source_side_census(altar_source *h, int C, long long *counts, void *stream)
+{
+    return guarded([&] {
+        require(h != nullptr && counts != nullptr && C > 0, "bad argument");
+        require(h->model == ALTAR_CDM, "side census: a CDM source is required");
+        require(h->flags.p != nullptr && (size_t)C <= h->flags.bytes / sizeof(unsigned),
+                "side census: no forward call has prepared this many chains");
+        hipStream_t st = S(stream);
+        h->census.reserve(4 * sizeof(unsigned long long));
+        hip_check(launch_side_census(h->flags.as<unsigned>(), C, h->census.as<unsigned long long>(), st), "census");
+        unsigned long long host[4];
+        hip_check(hipMemcpyAsync(host, h->census.p, sizeof(host), hipMemcpyDeviceToHost, st), "census copy");
+        hip_check(hipStreamSynchronize(st), "census sync");
+        for (int i = 0; i < 4; ++i) counts[i] = (long long)host[i];
+    });
+}
 
 int altar_source_timing(altar_source *h, int enable)
 {

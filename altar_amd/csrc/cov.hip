@@ -80,7 +80,9 @@ __device__ double radix_select(const double *__restrict__ x, int n, int k, unsig
         }
         __syncthreads();
     }
-    return key_f64(state[0]);
+    const double r = key_f64(state[0]);
+    __syncthreads();  // every thread has read state before a next call resets it
+    return r;
 }
 
 // COV of w = exp(dbeta (llk - median)) normalized to unit sum; w written

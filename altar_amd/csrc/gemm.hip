@@ -107,14 +107,18 @@ __device__ inline void gemm_epilogue(floatx16 (&acc)[2][2], const GemmEpi &ep, i
     }
 }
 
-// Block (bx, by) -> (n tile, m tile).  For a lower-triangular B the cost of
-// an n tile grows with its index; launching the expensive tiles first
-// shortens the tail.
+// Block -> (n tile, m tile) on a 1-D grid, m fastest.  Consecutive blocks
+// go round-robin to the 8 XCDs, so with the m index fastest each XCD gets
+// the same mix of n tiles (when ntm is a multiple of 8, XCD x owns the m
+// tiles congruent to x mod 8, for every n).  For a lower-triangular B the cost of an n tile grows with
+// its index: the expensive tiles go first (longest-first order), which keeps
+// the tail short, and every XCD carries the same share of every cost class.
 __device__ inline void tile_coords(int tri, int ntn, int &tn, int &tm)
 {
-    tn = blockIdx.x;
-    tm = blockIdx.y;
-    if (tri) tn = ntn - 1 - tn;
+    const int ntm = (int)gridDim.x / ntn;
+    const int cls = (int)blockIdx.x / ntm;
+    tm = (int)blockIdx.x - cls * ntm;
+    tn = tri ? ntn - 1 - cls : cls;
 }
 
 // ---------------------------------------------------------------- FP32
@@ -124,14 +128,14 @@ constexpr int LDS32 = BK32 + 4;  // row pitch (floats): conflict-free 16-byte re
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, size_t lda,
                                                    const float *__restrict__ B, size_t ldb, int Kp,
-                                                   int tri, GemmEpi ep)
+                                                   int ntn, int tri, GemmEpi ep)
 {
     __shared__ __attribute__((aligned(16))) float As[2][BM * LDS32];
     __shared__ __attribute__((aligned(16))) float Bs[2][BN * LDS32];
     __shared__ float red[2 * BM];
 
     int tn, tm;
-    tile_coords(tri, gridDim.x, tn, tm);
+    tile_coords(tri, ntn, tn, tm);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
@@ -215,7 +219,7 @@ template <int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__ Ahi, const __bf16 *__restrict__ Alo,
                                                       size_t lda, const __bf16 *__restrict__ Bhi,
                                                       const __bf16 *__restrict__ Blo, size_t ldb, int Kp,
-                                                      int tri, int plain, GemmEpi ep)
+                                                      int ntn, int tri, int plain, GemmEpi ep)
 {
     __shared__ __attribute__((aligned(16))) __bf16 Ash[2][BM * LDS16];
     __shared__ __attribute__((aligned(16))) __bf16 Asl[2][BM * LDS16];
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
     __shared__ float red[2 * BM];
 
     int tn, tm;
-    tile_coords(tri, gridDim.x, tn, tm);
+    tile_coords(tri, ntn, tn, tm);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
@@ -359,15 +363,16 @@ __global__ void whiten_f64_kernel(const double *__restrict__ R, size_t ldr, cons
 hipError_t launch_gemm_f32(const float *A, size_t lda, const float *B, size_t ldb, int Mp, int Np, int Kp,
                            int tri, int epi, const GemmEpi &ep, hipStream_t st)
 {
-    dim3 grid(Np / BN, Mp / BM), block(256);
+    const int ntn = Np / BN;
+    dim3 grid(ntn * (Mp / BM)), block(256);
     if (epi == EPI_STORE)
-        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE>), grid, block, 0, st, A, lda, B, ldb, Kp, ntn, tri, ep);
     else if (epi == EPI_STORE_SUMSQ)
-        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_STORE_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, ntn, tri, ep);
     else if (epi == EPI_SUMSQ)
-        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ>), grid, block, 0, st, A, lda, B, ldb, Kp, ntn, tri, ep);
     else
-        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ_BIAS>), grid, block, 0, st, A, lda, B, ldb, Kp, tri, ep);
+        hipLaunchKernelGGL((gemm_nt_f32<EPI_SUMSQ_BIAS>), grid, block, 0, st, A, lda, B, ldb, Kp, ntn, tri, ep);
     return hipGetLastError();
 }
 
@@ -375,17 +380,18 @@ hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, cons
                               size_t ldb, int Mp, int Np, int Kp, int tri, int plain, int epi,
                               const GemmEpi &ep, hipStream_t st)
 {
-    dim3 grid(Np / BN, Mp / BM), block(256);
+    const int ntn = Np / BN;
+    dim3 grid(ntn * (Mp / BM)), block(256);
     const __bf16 *ah = (const __bf16 *)Ahi, *al = (const __bf16 *)Alo;
     const __bf16 *bh = (const __bf16 *)Bhi, *bl = (const __bf16 *)Blo;
     if (epi == EPI_STORE)
-        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
     else if (epi == EPI_STORE_SUMSQ)
-        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_STORE_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
     else if (epi == EPI_SUMSQ)
-        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
     else
-        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ_BIAS>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, tri, plain, ep);
+        hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ_BIAS>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
     return hipGetLastError();
 }
 

@@ -165,5 +165,7 @@ hipError_t launch_cov_resample_src(const double *w, int C, uint64_t seed, uint32
                                    int *offset, int *src, hipStream_t st);
 
 hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int *mask, hipStream_t st);
+// out[0..3] = FP32 sides, FP64 sides, skipped sides of the valid chains, invalid chains
+hipError_t launch_side_census(const unsigned *flags, int C, unsigned long long *out, hipStream_t st);
 
 }  // namespace altar

@@ -293,4 +293,38 @@ hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int
     return hipGetLastError();
 }
 
+// How the sides of the prepared chains are evaluated (one workgroup; the
+// flags word of each chain as built by make_geometry).
+__global__ __launch_bounds__(256) void side_census_kernel(const unsigned *__restrict__ flags, int C,
+                                                          unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long acc[4];
+    if (threadIdx.x < 4) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    unsigned long long f32 = 0, f64 = 0, skip = 0, bad = 0;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        const unsigned f = flags[c];
+        if (f & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) {
+            ++bad;
+            continue;
+        }
+        const unsigned sk = f & 0xfffu, d = (f >> 12) & 0xfffu & ~sk;
+        skip += __popc(sk);
+        f64 += __popc(d);
+        f32 += 12 - __popc(sk) - __popc(d);
+    }
+    atomicAdd(&acc[0], f32);
+    atomicAdd(&acc[1], f64);
+    atomicAdd(&acc[2], skip);
+    atomicAdd(&acc[3], bad);
+    __syncthreads();
+    if (threadIdx.x < 4) out[threadIdx.x] = acc[threadIdx.x];
+}
+
+hipError_t launch_side_census(const unsigned *flags, int C, unsigned long long *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(side_census_kernel, dim3(1), dim3(256), 0, st, flags, C, out);
+    return hipGetLastError();
+}
+
 }  // namespace altar

@@ -74,6 +74,17 @@ class Source:
                   _lib.PRECISIONS[precision], _lib.stream())
         return out
 
+    def fp64_sides(self, threshold):
+        """CDM: sides with |sin(beta)| below threshold are evaluated in FP64."""
+        _lib.call("altar_source_fp64_sides", self.handle, float(threshold))
+
+    def side_census(self, C):
+        """CDM: (FP32 sides, FP64 sides, skipped sides, invalid chains) of the
+        C chains prepared by the last call."""
+        counts = (ctypes.c_longlong * 4)()
+        _lib.call("altar_source_side_census", self.handle, int(C), counts, _lib.stream())
+        return tuple(counts)
+
     def verify(self, theta, mask):
         theta = theta.contiguous()
         _lib.call("altar_source_verify", self.handle, _lib.ptr(theta), theta.shape[0], _lib.ptr(mask),

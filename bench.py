@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--chains", type=int, default=16384)
     ap.add_argument("--obs", type=int, default=5000)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
+    ap.add_argument("--fp64-sides", type=float, default=0.4,
+                    help="CDM sides with |sin(beta)| below this are evaluated in FP64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate delayed-acceptance leg")
@@ -66,26 +68,31 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
-def cpu_baseline(case, theta, seconds):
-    """FP64 oracle (oracle/oracle.c) on one core: CDM residuals + whitened L2."""
+def cpu_baseline(case, theta, seconds, omp=False):
+    """FP64 oracle (oracle/oracle.c): CDM residuals + whitened L2 on a bounded
+    sample of the chains; one thread, or the OpenMP build over chains."""
     from oracle import oracle as o
     from altar_amd import norms
     layout = (0, 2, 3, 4, 7, -1)
     W = o.exact_whitener(case["cd"])
     norm = norms.normalization(case["cd"])
     args = (case["x"], case["y"], case["los"], case["d"], case["oid"], 0.25)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1) if omp else 1
 
     def run(n):
         t0 = time.perf_counter()
-        r = o.cdm_residuals(theta[:n], layout, *args)
-        o.l2_llk(r, W, norm)
+        r = o.cdm_residuals(theta[:n], layout, *args, omp=omp)
+        o.l2_llk(r, W, norm, omp=omp)
         return time.perf_counter() - t0
 
-    t1 = run(1)
-    n = int(max(1, min(len(theta), seconds / max(t1, 1e-6))))
+    run(threads)                       # warm-up (thread pool, caches)
+    n1 = 8 * threads if omp else 2     # whole chunks of 4 chains for every thread
+    t1 = run(n1)
+    n = int(max(1, min(len(theta), seconds / max(t1, 1e-6) * n1)))
     t = run(n)
-    return {"value": n / t, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} chains x {len(case['x'])} obs, FP64 CDM + full-Cd whitening, 1 thread, {t:.1f} s"}
+    return {"value": n / t, "unit": "likelihood evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n} chains x {len(case['x'])} obs, FP64 CDM + full-Cd whitening, "
+                      f"{threads} thread{'s' if threads > 1 else ''}, {t:.1f} s"}
 
 
 def main():
@@ -110,7 +117,7 @@ def main():
     case = cdm_case(N=N)
     job = Job(chains=C, steps=1, seed=777 + 2)
     model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
-                nu=0.25, precision=args.precision, job=job)
+                nu=0.25, precision=args.precision, job=job, fp64_sides=args.fp64_sides)
     model.initialize()
     annealer = Annealer(Metropolis(), COV(), seed=job.seed).initialize(job)
     annealer.model = model
@@ -118,6 +125,8 @@ def main():
     worker.start(annealer)
     worker.cool(annealer)           # first beta stage: beta, Sigma, resampling
     step = worker.step
+    if not step.beta > 0.0:
+        raise RuntimeError(f"the first annealing stage did not advance (beta = {step.beta})")
     sampler = annealer.sampler
     sampler.prepareSamplingPDF(annealer, step)
     sampler.steps = args.warmup
@@ -138,6 +147,11 @@ def main():
     ms = (ctypes.c_double * 4)()
     cnt = (ctypes.c_longlong * 4)()
     _lib.call("altar_source_timers", src.handle, ms, cnt, 4)
+
+    f32s, f64s, skipped, invalid = src.side_census(C)
+    valid = max(C - invalid, 1)
+    sides = {"fp64_threshold": model.fp64_sides, "fp32_per_chain": f32s / valid,
+             "fp64_per_chain": f64s / valid, "invalid_chains": invalid}
 
     evals = C * args.steps
     value = evals / elapsed
@@ -180,12 +194,14 @@ def main():
         "accepted_samples_per_s": acc / elapsed,
         "acceptance": {"accepted": acc, "rejected": rej, "unlikely": unl},
         "kernels": kernels,
+        "cdm_sides": sides,
         "roofline": roofline,
         "setup_s": setup_s,
     }
     if not args.no_cpu:
         theta = step.theta.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(case, theta, args.cpu_seconds)
+        out["cpu_baseline_all_cores"] = cpu_baseline(case, theta, args.cpu_seconds / 2, omp=True)
     if not args.no_da:
         out["surrogate_da"] = da_leg(args, case, model, acc / elapsed)
     print(json.dumps(out), flush=True)

@@ -124,6 +124,17 @@ int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C
  * for Mogi and linear. */
 int altar_source_verify(altar_source *h, const double *theta_dev, int C, int *mask_dev, void *stream);
 
+/* CDM precision policy (no reference counterpart: the reference evaluates
+ * every side in FP64, models/cdm/cdm/libcdm.py:375-430).  Sides whose
+ * |sin(beta)| lies below threshold (default 0.4) are evaluated in FP64, the
+ * rest in FP32; 0 turns the FP64 sides off and any value above 1 (at most
+ * 2) evaluates every side in FP64. */
+int altar_source_fp64_sides(altar_source *h, double threshold);
+/* Census of the sides of the C chains prepared by the last CDM call:
+ * counts[0..3] = FP32 sides, FP64 sides, skipped (degenerate or switched-off
+ * rectangle) sides of the valid chains, and chains rejected as invalid. */
+int altar_source_side_census(altar_source *h, int C, long long *counts, void *stream);
+
 /* Per-kernel timing with HIP events recorded on the launch stream (no
  * reference counterpart; the reference's Profiler.py times wall clock).
  * Slots: 0 forward-model kernel, 1 whitening / linear GEMM.  altar_source_timers

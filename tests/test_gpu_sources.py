@@ -242,4 +242,41 @@ def test_fields_write_only_the_callers_rows(cuda, oracle):
     assert np.all(b[:C, 300:] == 0.0)
     ref = torch.empty(C, dtype=torch.float64, device=cuda)
     src.data_llk(t, ref, "fp32")
-    assert np.allclose(llk.cpu().numpy(), ref.cpu().numpy(), rtol=1e-6, atol=0)
+    # both are FP32 squared norms summed per tile: equal to FP32 rounding of
+    # the norm (normalization - llk), not of the llk, which can be near zero
+    a, r = llk.cpu().numpy(), ref.cpu().numpy()
+    assert np.all(np.abs(a - r) <= 1e-6 * (abs(norm) + np.abs(r)))
+
+
+def test_cdm_fp64_side_policy(cuda, oracle):
+    """The FP64-side threshold: 0 puts every side in FP32, 2 every side in
+    FP64 (then only the final FP32 rounding remains: <= 2e-6 normwise); the
+    census counts the sides of each kind."""
+    print("case: cdm fp64 side policy C=48 N=1500", flush=True)
+    rng = np.random.default_rng(14)
+    N = 1500
+    x = rng.uniform(-15e3, 15e3, N)
+    y = rng.uniform(-15e3, 15e3, N)
+    los = unit_los(N)
+    theta = cdm_theta(rng, 48, oracle)
+    layout = (0, 2, 3, 4, 7, -1)
+    ref = oracle.cdm_residuals(theta, layout, x, y, los, np.zeros(N), np.zeros(N, np.int32), 0.25)
+    src = Source(_lib.CDM, 10, N, 0.25)
+    src.layout([0, 2, 3, 4, 7, -1, 0])
+    src.observations(x, y, los, np.zeros(N), np.zeros(N, np.int32))
+    t = torch.from_numpy(theta).to(cuda)
+    errs, census = {}, {}
+    for thr in (0.0, 0.4, 2.0):
+        src.fp64_sides(thr)
+        got = src.forward(t, "fp32").cpu().numpy()
+        errs[thr] = (np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)).max()
+        census[thr] = src.side_census(48)
+    print("normwise err by threshold", errs, "census", census)
+    assert census[0.0][1] == 0 and census[2.0][0] == 0
+    assert census[0.0][3] == 0 and sum(census[0.0][:3]) == 12 * 48
+    for thr in (0.4, 2.0):
+        assert sum(census[thr][:3]) == 12 * 48 and census[thr][2] == census[0.0][2]
+    assert errs[2.0] < 2e-6
+    assert errs[0.4] < 2e-4
+    with pytest.raises(RuntimeError):
+        src.fp64_sides(-1.0)
