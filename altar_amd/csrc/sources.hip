@@ -180,8 +180,8 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                     double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
                     double e, nn, u;
                     cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya,
-                                           aa, H.ka2[a], ra, 1.0 / ra, 1.0 / (ra + aa), ab, H.ka2[b], rb,
-                                           1.0 / rb, 1.0 / (rb + ab), e, nn, u);
+                                           aa, H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa), ab, H.ka2[b],
+                                           rb, cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
                     due += e; dun += nn; duv += u;
                 } else {
                     float e, nn, u;
