@@ -30,10 +30,29 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic FP32 flops per (chain, observation) of the CDM forward + residual,
-# counted from altar_amd/csrc/cdm_math.hpp and sources.hip (fused multiply-add
-# = 2, divide/sqrt/log/atan2 = 1 each); see the census in DESIGN.md.
-CDM_FLOPS_PER_PAIR = 2719
+# Algorithmic FP32 operations per (chain, observation) of the CDM forward +
+# residual, counted from the source expressions of altar_amd/csrc/cdm_math.hpp
+# and sources.hip: add, subtract, multiply = 1 each (a fused multiply-add = 2),
+# reciprocal, square root, logarithm and atan2 = 1 each.  12 vertices x 11 +
+# 12 sides x 194 + 10 for the LOS projection and residual; 108 of them are
+# reciprocals, square roots, logarithms or atan2.
+CDM_FLOPS_PER_PAIR = 2470
+
+# HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) on the
+# default workload; lines "kernel <name> fetch_bytes <f> write_bytes <w>"
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_cdm5k_16384.txt")
+
+
+def measured_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) >= 6 and w[0] == "kernel" and w[1] == kernel:
+                    return int(w[3]) + int(w[5])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -43,6 +62,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chains", type=int, default=16384)
     ap.add_argument("--obs", type=int, default=5000)
+    ap.add_argument("--workload", default="cdm", choices=["cdm", "linear", "mogi"],
+                    help="cdm: the headline case; linear: 4,000 x 400 static slip, full Cd; "
+                         "mogi: 1,000-point point source, diagonal Cd, annealed to beta = 1")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
     ap.add_argument("--fp64-sides", type=float, default=0.4,
                     help="CDM sides with |sin(beta)| below this are evaluated in FP64")
@@ -104,6 +126,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(chain-parallel sharding is described in DESIGN.md, not built)"}))
         return 2
+    if args.workload != "cdm":
+        print(json.dumps(side_workload(args)), flush=True)
+        return 0
     import torch
     from altar_amd import Job, _lib
     from altar_amd.bayesian import Annealer, COV, Metropolis
@@ -170,8 +195,9 @@ def main():
     }
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     kd = kernels[dom]
-    # whitening: bytes the residual matrix and W occupy per launch (lower bound of HBM traffic)
-    traffic = None
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # counter summary of this workload (FP32 default precision only)
+    traffic = measured_traffic(dom) if args.precision == "fp32" and C == 16384 and N == 5000 else None
     roofline = {"bound": kd["bound"], "achieved": kd["tflops"], "peak": kd["peak"], "unit": "TFLOP/s",
                 "frac": kd["tflops"] / kd["peak"] if kd["tflops"] else None, "traffic": traffic,
                 "kernel": dom}
@@ -206,6 +232,94 @@ def main():
         out["surrogate_da"] = da_leg(args, case, model, acc / elapsed)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def side_workload(args):
+    """The other two cases, same timing rules (sample state resident, K timed
+    Metropolis steps at the first annealing temperature):
+    * linear: G 4,000 x 400, full exponential Cd, 16,384 chains; the
+      likelihood is one pre-whitened GEMM |W G theta - W d|^2 (FP32 or split
+      BF16 + refinement);
+    * mogi: 1,000 points, diagonal Cd, 1,024 chains, FP64; also the wall time
+      of the whole anneal to beta = 1 (5 steps per stage)."""
+    import torch
+    from altar_amd import Job, _lib
+    from altar_amd.bayesian import Annealer, COV, Metropolis
+    from altar_amd import cases
+
+    torch.cuda.init()
+    if args.workload == "linear":
+        from altar_amd.linear import Linear
+        C = args.chains
+        case = cases.linear_case()
+        job = Job(chains=C, steps=1, seed=777 + 1)
+        model = Linear(case["psets"], case["G"], case["d"], case["cd"], precision=args.precision, job=job)
+        N, P = case["G"].shape
+        flops_per_eval = 2 * N * P
+        name = f"linear static slip, {C:,} chains x {N:,} obs x {P} params, full Cd (pre-whitened GEMM)"
+    else:
+        from altar_amd.mogi import Mogi
+        C = 1024
+        case = cases.mogi_case()
+        job = Job(chains=C, steps=5, seed=777 + 0)
+        model = Mogi(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
+                     precision="fp64", job=job)
+        N, P = len(case["x"]), 5
+        flops_per_eval = None
+        name = f"mogi point source, {C:,} chains x {N:,} obs, diagonal Cd, FP64"
+    model.initialize()
+    ann = Annealer(Metropolis(), COV(), seed=job.seed).initialize(job)
+    ann.model = model
+    w = ann.worker
+    t_setup = time.perf_counter()
+    w.start(ann)
+    w.cool(ann)
+    step = w.step
+    if not step.beta > 0.0:
+        raise RuntimeError(f"the first annealing stage did not advance (beta = {step.beta})")
+    sampler = ann.sampler
+    sampler.prepareSamplingPDF(ann, step)
+    sampler.steps = args.warmup
+    sampler.walkChains(ann, step)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t_setup
+    src = model.source
+    _lib.call("altar_source_timing", src.handle, 1)
+    sampler.steps = args.steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    acc, rej, unl = sampler.walkChains(ann, step)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("altar_source_timing", src.handle, 0)
+    import ctypes
+    ms = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_longlong * 4)()
+    _lib.call("altar_source_timers", src.handle, ms, cnt, 4)
+    out = {"metric": "likelihood evals/s", "value": C * args.steps / elapsed, "unit": "likelihood evals/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp64" if args.workload == "mogi" else args.precision,
+           "data": "synthetic (seeded)", "config": {"workload": name, "beta": step.beta},
+           "accepted_samples_per_s": acc / elapsed, "acceptance": {"accepted": acc, "rejected": rej,
+                                                                  "unlikely": unl},
+           "setup_s": setup}
+    if cnt[1]:
+        g = ms[1] / cnt[1]
+        out["kernels"] = {"gemm": {"avg_ms": g, "tflops": C * flops_per_eval / (g * 1e-3) / 1e12
+                                   if flops_per_eval else None}}
+    if cnt[0]:
+        out.setdefault("kernels", {})["forward"] = {"avg_ms": ms[0] / cnt[0]}
+    if args.workload == "mogi":
+        # the whole anneal from a fresh sample
+        ann2 = Annealer(Metropolis(steps=5), COV(), seed=job.seed).initialize(job)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ann2.posterior(model)
+        torch.cuda.synchronize()
+        out["anneal"] = {"wall_s": time.perf_counter() - t1, "stages": len(ann2.history),
+                         "final_beta": ann2.worker.beta}
+    return out
 
 
 def da_leg(args, case, model, plain_acc_rate):
