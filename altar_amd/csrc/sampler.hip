@@ -37,7 +37,8 @@ __global__ void normals_kernel(double *__restrict__ xi, int C, int P, uint64_t s
     if (2 * w + 1 < P) xi[(size_t)c * P + 2 * w + 1] = rad * s;
 }
 
-// candidate = theta + (L xi)^T, one thread per (chain, parameter); L lower
+// candidate = theta + (L xi)^T, one thread per (chain, parameter); L lower.
+// For small P (the point-source and CDM models).
 __global__ void propose_kernel(const double *__restrict__ theta, const double *__restrict__ L,
                                const double *__restrict__ xi, int C, int P, double *__restrict__ cand)
 {
@@ -49,6 +50,97 @@ __global__ void propose_kernel(const double *__restrict__ theta, const double *_
     double acc = 0.0;
     for (int j = 0; j <= i; ++j) acc += l[j] * x[j];
     cand[idx] = theta[idx] + acc;
+}
+
+// The same product for large P (the linear model), tiled through LDS: a
+// block computes 128 chains x 64 parameters, each thread 8 consecutive
+// chains x 4 consecutive parameters, over k in slices of 16 (the next slice
+// is fetched into registers while the current one is multiplied).  Every
+// output sums k = 0, 1, ..., i in order with fused multiply-adds, so it
+// differs from the small-P kernel by rounding only; slices below the tile's
+// diagonal skip the k <= i test.
+constexpr int PTM = 128, PTN = 64, PK = 16;
+__global__ __launch_bounds__(256) void propose_tiled_kernel(const double *__restrict__ theta,
+                                                            const double *__restrict__ L,
+                                                            const double *__restrict__ xi, int C, int P,
+                                                            double *__restrict__ cand)
+{
+    // pitch PTM + 1 / PTN + 1: the 16 k of one row land in distinct banks
+    __shared__ double Xs[PK][PTM + 1];
+    __shared__ double Ls[PK][PTN + 1];
+    const int m0 = blockIdx.x * PTM, n0 = blockIdx.y * PTN;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    const int kmax = min(P, n0 + PTN);  // L is lower triangular: k <= n < n0 + PTN
+    // staging: thread t moves (row r, k) = (t / 16 + 16 q, t % 16), q < 8 for
+    // xi and q < 4 for L: 16 consecutive k of a row are one 128-byte line
+    const int sk = threadIdx.x & 15, sr = threadIdx.x >> 4;
+    double rx[8], rl[4];
+    auto fetch = [&](int k0) {
+        const int k = k0 + sk;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int m = m0 + sr + 16 * q;
+            rx[q] = (m < C && k < P) ? xi[(size_t)m * P + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n = n0 + sr + 16 * q;
+            rl[q] = (n < P && k < P) ? L[(size_t)n * P + k] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < kmax; k0 += PK) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Xs[sk][sr + 16 * q] = rx[q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ls[sk][sr + 16 * q] = rl[q];
+        __syncthreads();
+        if (k0 + PK < kmax) fetch(k0 + PK);
+        const int kn = min(PK, kmax - k0);
+        if (k0 + PK <= n0) {
+            // the whole slice lies at or below the diagonal of every column
+            for (int kk = 0; kk < kn; ++kk) {
+                double x[8], l[4];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) x[i] = Xs[kk][ty * 8 + i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) l[j] = Ls[kk][tx * 4 + j];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = fma(l[j], x[i], acc[i][j]);
+            }
+        } else {
+            for (int kk = 0; kk < kn; ++kk) {
+                const int k = k0 + kk;
+                double x[8], l[4];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) x[i] = Xs[kk][ty * 8 + i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) l[j] = k <= n0 + tx * 4 + j ? Ls[kk][tx * 4 + j] : 0.0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = fma(l[j], x[i], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int m = m0 + ty * 8 + i;
+        if (m >= C) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + tx * 4 + j;
+            if (n < P) cand[(size_t)m * P + n] = theta[(size_t)m * P + n] + acc[i][j];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- priors
@@ -86,6 +178,43 @@ __global__ void prior_llk_kernel(const double *__restrict__ theta, int C, int P,
         }
     }
     llk[c] = acc;
+}
+
+// The prior for long parameter vectors: one wavefront per chain, lanes over
+// the parameters, a fixed-order butterfly sum (deterministic).
+__global__ __launch_bounds__(256) void prior_llk_wave_kernel(const double *__restrict__ theta, int C, int P,
+                                                             const int *__restrict__ kind,
+                                                             const double *__restrict__ a,
+                                                             const double *__restrict__ b, double *__restrict__ llk)
+{
+    const int c = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (c >= C) return;  // uniform over the wavefront
+    const double *t = theta + (size_t)c * P;
+    double acc = 0.0;
+    for (int j = lane; j < P; j += 64) {
+        if (kind[j] == 0) {
+            acc += log(1.0 / (b[j] - a[j]));
+        } else {
+            double z = (t[j] - a[j]) / b[j];
+            acc += -0.5 * z * z - log(b[j] * sqrt(2.0 * PI));
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) llk[c] = acc;
+}
+
+__global__ __launch_bounds__(256) void prior_verify_wave_kernel(const double *__restrict__ theta, int C, int P,
+                                                                const int *__restrict__ kind,
+                                                                const double *__restrict__ a,
+                                                                const double *__restrict__ b, int *__restrict__ mask)
+{
+    const int c = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (c >= C) return;
+    const double *t = theta + (size_t)c * P;
+    int bad = 0;
+    for (int j = lane; j < P; j += 64)
+        if (kind[j] == 0 && !(a[j] <= t[j] && t[j] <= b[j])) bad = 1;
+    if (__any(bad) && lane == 0) mask[c] += 1;
 }
 
 // initial sample from the prep distributions, one thread per (chain, param)
@@ -154,13 +283,26 @@ __global__ void accept_kernel(double *__restrict__ theta, double *__restrict__ p
                 dec = 2;
             } else {
                 dec = 0;
-                for (int j = 0; j < P; ++j) theta[(size_t)c * P + j] = ctheta[(size_t)c * P + j];
+                if (P <= 32)
+                    for (int j = 0; j < P; ++j) theta[(size_t)c * P + j] = ctheta[(size_t)c * P + j];
                 prior[c] = cprior[c];
                 data[c] = cdata[c];
                 post[c] = cp;
             }
         }
         if (decision) decision[c] = dec;
+    }
+    if (P > 32) {
+        // long rows: the block copies each accepted row with all its threads
+        __shared__ int sdec[256];
+        sdec[threadIdx.x] = dec;
+        __syncthreads();
+        const size_t c0 = (size_t)blockIdx.x * blockDim.x;
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            if (sdec[k] != 0) continue;
+            const size_t base = (c0 + k) * P;
+            for (int j = threadIdx.x; j < P; j += blockDim.x) theta[base + j] = ctheta[base + j];
+        }
     }
     // block-level counts, then one atomic per counter per block
     __shared__ unsigned cnt[3];
@@ -183,38 +325,101 @@ __global__ void posterior_kernel(const double *__restrict__ prior, const double 
     }
 }
 
-// weighted mean of every parameter (COV.py:137-140): one thread per parameter
-__global__ void wmean_kernel(const double *__restrict__ theta, const double *__restrict__ w, int C, int P,
-                             double *__restrict__ mean)
+// Weighted moments of the sample (COV.py:141-152): mean[j] = sum_i w_i
+// theta_ij / sum_i w_i and Sigma_ab = sum_i w_i theta_ia theta_ib -
+// mean_a mean_b.  Every sum runs over the chains in order (the host
+// restatement's order, so results agree bit for bit); the blocks stage
+// chunks of the sample through LDS with coalesced loads and each output is
+// one thread's running sum.
+constexpr int WCH = 64;  // chains per LDS round
+
+// one block per 64 parameters; thread t stages (chain t / 64 + 4 q, param
+// t % 64) for q < 16, the next chunk fetched while the current one is summed
+__global__ __launch_bounds__(256) void wmean_kernel(const double *__restrict__ theta, const double *__restrict__ w,
+                                                    int C, int P, double *__restrict__ mean)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P) return;
-    double sw = 0.0, acc = 0.0;
-    for (int i = 0; i < C; ++i) {
-        sw += w[i];
-        acc += w[i] * theta[(size_t)i * P + j];
+    __shared__ double st[WCH][64 + 1];
+    __shared__ double sw[WCH];
+    const int p0 = blockIdx.x * 64, j = p0 + (int)threadIdx.x;
+    const int sq = threadIdx.x & 63, sr = threadIdx.x >> 6;
+    double rt[WCH / 4], rw = 0.0;
+    auto fetch = [&](int i0) {
+#pragma unroll
+        for (int q = 0; q < WCH / 4; ++q) {
+            const int i = i0 + sr + 4 * q;
+            rt[q] = (i < C && p0 + sq < P) ? theta[(size_t)i * P + p0 + sq] : 0.0;
+        }
+        if (threadIdx.x < WCH) rw = i0 + (int)threadIdx.x < C ? w[i0 + threadIdx.x] : 0.0;
+    };
+    double wsum = 0.0, acc = 0.0;
+    fetch(0);
+    for (int i0 = 0; i0 < C; i0 += WCH) {
+#pragma unroll
+        for (int q = 0; q < WCH / 4; ++q) st[sr + 4 * q][sq] = rt[q];
+        if (threadIdx.x < WCH) sw[threadIdx.x] = rw;
+        __syncthreads();
+        if (i0 + WCH < C) fetch(i0 + WCH);
+        if (threadIdx.x < 64) {
+            const int n = min(WCH, C - i0);
+#pragma unroll 16
+            for (int r = 0; r < n; ++r) {
+                wsum += sw[r];
+                acc += sw[r] * st[r][threadIdx.x];
+            }
+        }
+        __syncthreads();
     }
-    mean[j] = acc / sw;
+    if (threadIdx.x < 64 && j < P) mean[j] = acc / wsum;
 }
 
-// Sigma = sum_i w_i theta_i theta_i^T - mean mean^T (COV.py:141-152), one
-// thread per lower-triangle entry, fixed summation order
-__global__ void wcov_kernel(const double *__restrict__ theta, const double *__restrict__ w,
-                            const double *__restrict__ mean, int C, int P, double *__restrict__ sigma)
+// one block per 16 x 16 tile of the lower triangle (tile row ta >= column
+// tb); thread t stages (chain t / 16 + 16 q, column t % 16) of both column
+// blocks for q < 4, the next chunk fetched while the current one is summed
+__global__ __launch_bounds__(256) void wcov_kernel(const double *__restrict__ theta, const double *__restrict__ w,
+                                                   const double *__restrict__ mean, int C, int P,
+                                                   double *__restrict__ sigma)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long ne = (long)P * (P + 1) / 2;
-    if (e >= ne) return;
-    // e -> (a, b) with a >= b
-    int a = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
-    while ((long)a * (a + 1) / 2 > e) --a;
-    while ((long)(a + 1) * (a + 2) / 2 <= e) ++a;
-    const int b = (int)(e - (long)a * (a + 1) / 2);
+    __shared__ double sa[WCH][16 + 1], sb[WCH][16 + 1];
+    __shared__ double sw[WCH];
+    const int t = blockIdx.x;
+    int ta = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
+    while (ta * (ta + 1) / 2 > t) --ta;
+    while ((ta + 1) * (ta + 2) / 2 <= t) ++ta;
+    const int tb = t - ta * (ta + 1) / 2;
+    const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const int a = ta * 16 + r, b = tb * 16 + c;
+    const int ca = ta * 16 + c, cb = tb * 16 + c;  // staged columns (c doubles as the column slot)
+    double ra[WCH / 16], rb[WCH / 16], rw = 0.0;
+    auto fetch = [&](int i0) {
+#pragma unroll
+        for (int q = 0; q < WCH / 16; ++q) {
+            const int i = i0 + r + 16 * q;
+            ra[q] = (i < C && ca < P) ? theta[(size_t)i * P + ca] : 0.0;
+            rb[q] = (i < C && cb < P) ? theta[(size_t)i * P + cb] : 0.0;
+        }
+        if (threadIdx.x < WCH) rw = i0 + (int)threadIdx.x < C ? w[i0 + threadIdx.x] : 0.0;
+    };
     double acc = 0.0;
-    for (int i = 0; i < C; ++i) acc += w[i] * theta[(size_t)i * P + a] * theta[(size_t)i * P + b];
-    const double v = acc - mean[a] * mean[b];
-    sigma[(size_t)a * P + b] = v;
-    sigma[(size_t)b * P + a] = v;
+    fetch(0);
+    for (int i0 = 0; i0 < C; i0 += WCH) {
+#pragma unroll
+        for (int q = 0; q < WCH / 16; ++q) {
+            sa[r + 16 * q][c] = ra[q];
+            sb[r + 16 * q][c] = rb[q];
+        }
+        if (threadIdx.x < WCH) sw[threadIdx.x] = rw;
+        __syncthreads();
+        if (i0 + WCH < C) fetch(i0 + WCH);
+        const int n = min(WCH, C - i0);
+#pragma unroll 16
+        for (int ii = 0; ii < n; ++ii) acc += sw[ii] * sa[ii][r] * sb[ii][c];
+        __syncthreads();
+    }
+    if (a < P && b < P && a >= b) {
+        const double v = acc - mean[a] * mean[b];
+        sigma[(size_t)a * P + b] = v;
+        sigma[(size_t)b * P + a] = v;
+    }
 }
 
 // resampled state: row c takes old row src[c]
@@ -249,21 +454,33 @@ hipError_t launch_normals(double *xi, int C, int P, uint64_t seed, uint32_t stag
 hipError_t launch_propose(const double *theta, const double *L, const double *xi, int C, int P, double *cand,
                           hipStream_t st)
 {
-    hipLaunchKernelGGL(propose_kernel, grid1((long)C * P), dim3(256), 0, st, theta, L, xi, C, P, cand);
+    if (P <= 32)
+        hipLaunchKernelGGL(propose_kernel, grid1((long)C * P), dim3(256), 0, st, theta, L, xi, C, P, cand);
+    else
+        hipLaunchKernelGGL(propose_tiled_kernel, dim3((C + PTM - 1) / PTM, (P + PTN - 1) / PTN), dim3(256), 0,
+                           st, theta, L, xi, C, P, cand);
     return hipGetLastError();
 }
 
 hipError_t launch_prior_verify(const double *theta, int C, int P, const int *kind, const double *a,
                                const double *b, int *mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(prior_verify_kernel, grid1(C), dim3(256), 0, st, theta, C, P, kind, a, b, mask);
+    if (P <= 32)
+        hipLaunchKernelGGL(prior_verify_kernel, grid1(C), dim3(256), 0, st, theta, C, P, kind, a, b, mask);
+    else
+        hipLaunchKernelGGL(prior_verify_wave_kernel, grid1((long)C * 64), dim3(256), 0, st, theta, C, P, kind, a,
+                           b, mask);
     return hipGetLastError();
 }
 
 hipError_t launch_prior_llk(const double *theta, int C, int P, const int *kind, const double *a, const double *b,
                             double *llk, hipStream_t st)
 {
-    hipLaunchKernelGGL(prior_llk_kernel, grid1(C), dim3(256), 0, st, theta, C, P, kind, a, b, llk);
+    if (P <= 32)
+        hipLaunchKernelGGL(prior_llk_kernel, grid1(C), dim3(256), 0, st, theta, C, P, kind, a, b, llk);
+    else
+        hipLaunchKernelGGL(prior_llk_wave_kernel, grid1((long)C * 64), dim3(256), 0, st, theta, C, P, kind, a, b,
+                           llk);
     return hipGetLastError();
 }
 
@@ -299,9 +516,9 @@ hipError_t launch_posterior(const double *prior, const double *data, double beta
 hipError_t launch_wcov(const double *theta, const double *w, int C, int P, double *mean, double *sigma,
                        hipStream_t st)
 {
-    hipLaunchKernelGGL(wmean_kernel, grid1(P, 64), dim3(64), 0, st, theta, w, C, P, mean);
-    hipLaunchKernelGGL(wcov_kernel, grid1((long)P * (P + 1) / 2, 64), dim3(64), 0, st, theta, w, mean, C, P,
-                       sigma);
+    const int nt = (P + 15) / 16;
+    hipLaunchKernelGGL(wmean_kernel, dim3((P + 63) / 64), dim3(256), 0, st, theta, w, C, P, mean);
+    hipLaunchKernelGGL(wcov_kernel, dim3(nt * (nt + 1) / 2), dim3(256), 0, st, theta, w, mean, C, P, sigma);
     return hipGetLastError();
 }
 

@@ -26,9 +26,13 @@ def test_normals_match_oracle(cuda, oracle):
     assert np.max(np.abs(out.cpu().numpy() - ref)) < 1e-12
 
 
-def test_propose_restore(cuda, oracle):
+@pytest.mark.parametrize("C,P", [(2048, 10), (1000, 100), (2048, 400)])
+def test_propose_restore(cuda, oracle, C, P):
+    """Proposals against the oracle: bit for bit given the same normals for
+    P <= 32; the tiled kernel for P > 32 uses fused multiply-adds (same order,
+    1e-14 of the row scale)."""
+    print(f"case: propose C={C} P={P}", flush=True)
     rng = np.random.default_rng(7)
-    C, P = 2048, 10
     theta = rng.normal(size=(C, P))
     A = rng.normal(size=(P, P))
     L = np.linalg.cholesky(A @ A.T + P * np.eye(P))
@@ -39,6 +43,12 @@ def test_propose_restore(cuda, oracle):
     _lib.call("altar_mh_propose", _lib.ptr(t), C, P, _lib.ptr(Ld), 99, 2, 4, _lib.ptr(xi), _lib.ptr(cand), None)
     ref = oracle.mh_propose(theta, L, oracle.normals(99, 2, 4, 0, C, P))
     assert np.max(np.abs(cand.cpu().numpy() - ref) / (1 + np.abs(ref))) < 1e-12
+    same = oracle.mh_propose(theta, L, xi.cpu().numpy())
+    if P <= 32:
+        assert np.array_equal(cand.cpu().numpy(), same)
+    else:
+        scale = np.abs(xi.cpu().numpy()) @ np.abs(L).T + np.abs(theta)
+        assert np.max(np.abs(cand.cpu().numpy() - same) / scale) < 1e-14
     mask = torch.from_numpy(rng.integers(0, 2, C).astype(np.int32)).to(cuda)
     _lib.call("altar_mh_restore", _lib.ptr(cand), _lib.ptr(t), _lib.ptr(mask), C, P, None)
     m = mask.cpu().numpy().astype(bool)
@@ -46,9 +56,10 @@ def test_propose_restore(cuda, oracle):
     assert np.array_equal(c[m], theta[m])
 
 
-def test_accept_bit_exact(cuda, oracle):
+@pytest.mark.parametrize("C,P", [(4096, 6), (1001, 150)])
+def test_accept_bit_exact(cuda, oracle, C, P):
+    print(f"case: accept C={C} P={P}", flush=True)
     rng = np.random.default_rng(8)
-    C, P = 4096, 6
     theta = rng.normal(size=(C, P))
     prior, data = rng.normal(size=C), rng.normal(size=C) * 10
     beta = 0.37
@@ -81,13 +92,16 @@ def test_accept_bit_exact(cuda, oracle):
             assert np.array_equal(a.cpu().numpy(), b)
 
 
-def test_prior_verify_llk_sample(cuda, oracle):
+@pytest.mark.parametrize("rep", [1, 40])
+def test_prior_verify_llk_sample(cuda, oracle, rep):
+    """P = 5 (one thread per chain) and P = 200 (one wavefront per chain)."""
     rng = np.random.default_rng(9)
-    C, P = 3000, 5
-    kind = np.array([0, 0, 1, 0, 1], dtype=np.int32)
-    a = np.array([-1.0, 0.0, 0.5, 10.0, -2.0])
-    b = np.array([1.0, 2.0, 2.0, 20.0, 0.1])
-    theta = rng.normal(size=(C, P)) * np.array([1.2, 1.5, 2, 6, 1]) + np.array([0, 1, 0, 15, 0])
+    C, P = 3000, 5 * rep
+    print(f"case: prior C={C} P={P}", flush=True)
+    kind = np.tile(np.array([0, 0, 1, 0, 1], dtype=np.int32), rep)
+    a = np.tile(np.array([-1.0, 0.0, 0.5, 10.0, -2.0]), rep)
+    b = np.tile(np.array([1.0, 2.0, 2.0, 20.0, 0.1]), rep)
+    theta = rng.normal(size=(C, P)) * np.tile([1.2, 1.5, 2, 6, 1], rep) + np.tile([0, 1, 0, 15, 0], rep)
     theta[0, 0] = 1.0  # the closed upper bound is inside (Uniform.py:66)
     ref_llk, ref_mask = oracle.prior(theta, kind, a, b)
     from altar_amd.model import Prior
@@ -102,14 +116,15 @@ def test_prior_verify_llk_sample(cuda, oracle):
     # initial draws stay inside the uniform supports
     _lib.call("altar_prior_sample", pr.handle, _lib.ptr(t), C, 11, 0, 0, None, None)
     s = t.cpu().numpy()
-    for j in (0, 1, 3):
+    for j in (0, 1, 3, P - 2):
         assert s[:, j].min() >= a[j] and s[:, j].max() < b[j]
     assert abs(s[:, 2].mean() - a[2]) < 5 * b[2] / np.sqrt(C)
 
 
-def test_covariance_and_gather(cuda, oracle):
+@pytest.mark.parametrize("C,P", [(5000, 8), (3001, 37)])
+def test_covariance_and_gather(cuda, oracle, C, P):
+    print(f"case: weighted covariance C={C} P={P}", flush=True)
     rng = np.random.default_rng(10)
-    C, P = 5000, 8
     theta = rng.normal(size=(C, P)) @ rng.normal(size=(P, P)) + rng.normal(size=P)
     w = rng.random(C)
     w /= w.sum()
@@ -121,6 +136,9 @@ def test_covariance_and_gather(cuda, oracle):
     rm, rs = oracle.weighted_covariance(theta, w)
     assert np.allclose(mean.cpu().numpy(), rm, rtol=1e-12, atol=1e-12)
     assert np.allclose(sig.cpu().numpy(), rs, rtol=1e-10, atol=1e-12)
+    # same summation order as the oracle: bit for bit on the lower triangle
+    assert np.array_equal(mean.cpu().numpy(), rm)
+    assert np.array_equal(np.tril(sig.cpu().numpy()), np.tril(rs))
     # closed form: numpy's weighted covariance with bias=True
     assert np.allclose(rs, np.cov(theta.T, aweights=w, bias=True), rtol=1e-9, atol=1e-12)
     src = rng.integers(0, C, C).astype(np.int32)
