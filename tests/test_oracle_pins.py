@@ -183,3 +183,44 @@ def test_accept_bookkeeping(oracle):
     assert np.array_equal(th[dec == 0], cth[dec == 0]) and np.array_equal(th[dec != 0], th0[dec != 0])
     assert np.isclose(oracle.mh_adjust_scaling(acc, (dec == 1).sum(), (dec == 2).sum()),
                       min(1, max(0.1, (8 * acc / C + 1) / 9)))
+
+
+def test_small_routines_against_library_forms(oracle):
+    """The oracle's short routines against numpy/scipy restatements of the
+    textbook forms: Box-Muller normals from the Philox uniform pairs, the
+    proposal theta + L xi, the linear residual G theta - d, the prior log
+    density (scipy.stats) and its closed-support verify, the initial draws."""
+    from scipy import stats
+    rng = np.random.default_rng(5)
+    C, P = 300, 7
+    z = oracle.normals(9, 2, 3, 0, C, P)
+    u = oracle.uniform_pairs(9, 2, 3, 0, C, (P + 1) // 2)
+    rad = np.sqrt(-2.0 * np.log(u[:, :, 0]))
+    bm = np.stack([rad * np.cos(2 * np.pi * u[:, :, 1]), rad * np.sin(2 * np.pi * u[:, :, 1])], -1)
+    assert np.allclose(z, bm.reshape(C, -1)[:, :P], rtol=1e-13, atol=1e-13)
+    theta = rng.normal(size=(C, P))
+    A = rng.normal(size=(P, P))
+    L = np.linalg.cholesky(A @ A.T + np.eye(P))
+    assert np.allclose(oracle.mh_propose(theta, L, z), theta + z @ L.T, rtol=1e-13, atol=1e-13)
+    G = rng.normal(size=(50, P))
+    d = rng.normal(size=50)
+    assert np.allclose(oracle.linear_residuals(theta, G, d), theta @ G.T - d, rtol=1e-13, atol=1e-13)
+    kinds = np.array([0, 1, 0, 1, 1, 0, 0], dtype=np.int32)
+    lo = np.array([-1.0, 0.5, 0.0, -2.0, 3.0, -5.0, 1.0])
+    hi = np.array([1.0, 2.0, 3.0, 0.1, 4.0, 5.0, 2.0])
+    theta = rng.uniform(-1, 1, (C, P)) * np.where(kinds == 0, (hi - lo) / 2 * 1.2, 3 * hi) + \
+        np.where(kinds == 0, (hi + lo) / 2, lo)
+    theta[0, 0] = 1.0   # upper bound of the closed support
+    llk, mask = oracle.prior(theta, kinds, lo, hi)
+    inside = np.all((kinds == 1) | ((theta >= lo) & (theta <= hi)), axis=1)
+    assert np.array_equal(mask == 0, inside)
+    dens = np.where(kinds == 0, stats.uniform.logpdf(np.clip(theta, lo, hi), lo, np.where(kinds == 0, hi - lo, 1)),
+                    stats.norm.logpdf(theta, lo, hi))
+    assert np.allclose(llk, dens.sum(axis=1), rtol=1e-12, atol=1e-12)
+    init = oracle.prep_sample(4, 0, kinds, lo, hi, 4000)
+    for j in np.where(kinds == 0)[0]:
+        assert init[:, j].min() >= lo[j] and init[:, j].max() < hi[j]
+    for j in np.where(kinds == 1)[0]:
+        assert abs(init[:, j].mean() - lo[j]) < 5 * hi[j] / np.sqrt(4000)
+        assert abs(init[:, j].std() / hi[j] - 1) < 0.05
+    assert oracle.median(np.array([3.0, 1.0, 2.0, 10.0])) == 2.5
