@@ -12,8 +12,8 @@ LIB_PATH = os.path.join(_HERE, "libaltar_amd.so")
 
 # status codes, models, whitening kinds, precisions (include/altar_amd.h)
 OK, EINVAL, EHIP, ENOMEM, ESTATE = 0, 1, 2, 3, 4
-MOGI, CDM, LINEAR = 0, 1, 2
-WHITEN_DIAG, WHITEN_LOWER = 1, 2
+MOGI, CDM, LINEAR, JOINT = 0, 1, 2, 3
+WHITEN_DIAG, WHITEN_LOWER, WHITEN_BLOCKS = 1, 2, 3
 FP64, FP32, BF16, BF16_REFINED = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16r": BF16_REFINED}
 
@@ -46,6 +46,7 @@ SIGNATURES = {
     "altar_source_observations": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "altar_source_green": (_i, [_vp, _vp, _vp, _i]),
     "altar_source_whitening": (_i, [_vp, _i, _vp, _d]),
+    "altar_source_whitening_blocks": (_i, [_vp, _i, _vp, _vp, _vp, _d]),
     "altar_source_forward": (_i, [_vp, _vp, _i, _vp, _sz, _i, _vp]),
     "altar_source_residuals": (_i, [_vp, _vp, _i, _vp, _sz, _i, _vp]),
     "altar_source_data_llk": (_i, [_vp, _vp, _i, _vp, _i, _vp]),

@@ -11,6 +11,12 @@
   (1.0, 1.2, 0.8) km, opening 0.1 m, sigma 8 mm with a full exponential Cd
   (sill 64 mm^2, range 5 km), 10 parameters.
 
+* ``joint_case``: a Mogi source and a CDM observed by an ascending and a
+  descending InSAR track (20,000 points each on offset lattices over
+  40 x 40 km; exponential Cd, sill 48 mm^2, range 5 km, plus a 16 mm^2
+  nugget) and 200 three-component GNSS stations (sigma 2 mm horizontal,
+  5 mm vertical, diagonal); 14 source parameters and 3 dataset offsets.
+
 Each returns a dict with observations, covariance, parameter sets and the
 true parameter vector.  The CDM data are computed by the given forward model
 (default: this package's GPU kernel).
@@ -101,5 +107,64 @@ def gpu_cdm_forward(theta, x, y, los, nu=0.25):
     src = Source(_lib.CDM, theta.shape[1], N, nu)
     src.layout([0, 2, 3, 4, 7, -1, 0])
     src.observations(x, y, los, np.zeros(N), np.zeros(N, np.int32))
+    t = torch.from_numpy(np.ascontiguousarray(theta, dtype=np.float64)).cuda()
+    return src.forward(t, "fp32").cpu().numpy()
+
+
+LOS_ASC = np.array([-0.60, -0.11, 0.79])
+LOS_DESC = np.array([0.60, -0.11, 0.79])
+JOINT_TRUTH = np.array([-6000.0, 4000.0, 4000.0, 6.7,                       # Mogi x, y, depth, log10 dV
+                        5000.0, -3000.0, 4000.0, 0.1, 1000.0, 1200.0, 800.0,  # CDM x, y, depth, opening, a
+                        5.0, -8.0, 30.0,                                      # CDM omega
+                        0.01, -0.02, 0.0])                                    # dataset offsets
+
+
+def joint_case(n_insar=20000, n_gnss=200, seed=20261207, forward=None):
+    """Datasets (ascending, descending, GNSS) and truth of the joint case.
+    ``forward(theta (1 x 17), x, y, los, oid) -> u (N)`` gives the noise-free
+    prediction (default: this package's GPU kernels)."""
+    from .joint import Dataset, default_psets
+    rng = np.random.default_rng(seed)
+    nx = int(np.ceil(np.sqrt(2 * n_insar)))
+    ny = int(np.ceil(n_insar / nx))
+    gx, gy = np.linspace(-20e3, 20e3, nx), np.linspace(-20e3, 20e3, ny)
+    X, Y = np.meshgrid(gx, gy)
+    xa, ya = X.ravel()[:n_insar], Y.ravel()[:n_insar]
+    xd, yd = xa + 0.5 * (gx[1] - gx[0]), ya + 0.5 * (gy[1] - gy[0])
+    sg = rng.uniform(-25e3, 25e3, (n_gnss, 2))
+    xg, yg = np.repeat(sg[:, 0], 3), np.repeat(sg[:, 1], 3)
+    lg = np.tile(np.eye(3), (n_gnss, 1))
+    parts = [(xa, ya, np.tile(LOS_ASC / np.linalg.norm(LOS_ASC), (n_insar, 1))),
+             (xd, yd, np.tile(LOS_DESC / np.linalg.norm(LOS_DESC), (n_insar, 1))),
+             (xg, yg, lg)]
+    truth = JOINT_TRUTH.copy()
+    if forward is None:
+        forward = gpu_joint_forward
+    datasets = []
+    for k, (x, y, los) in enumerate(parts):
+        u = np.asarray(forward(truth[None, :], x, y, los, np.full(len(x), k, np.int32))).reshape(-1)
+        chol = None
+        if k < 2:
+            cd = norms.exponential_covariance(np.stack([x, y], 1), 48e-6, 5e3, nugget=16e-6)
+            chol = np.linalg.cholesky(cd)
+            noise = chol @ rng.standard_normal(len(x))
+        else:
+            cd = np.tile([4e-6, 4e-6, 25e-6], n_gnss)
+            noise = np.sqrt(cd) * rng.standard_normal(len(x))
+        datasets.append(Dataset(x=x, y=y, los=los, d=u + noise, cd=cd, name=("asc", "desc", "gnss")[k],
+                                chol=chol))
+    return dict(datasets=datasets, psets=default_psets(), truth=truth, nu=0.25)
+
+
+def gpu_joint_forward(theta, x, y, los, oid, nu=0.25):
+    """Mogi + CDM prediction minus the dataset shift (parameters in the
+    joint layout), on the GPU."""
+    import torch
+    from . import _lib
+    from .source import Source
+    N = len(x)
+    src = Source(_lib.JOINT, theta.shape[1], N, nu)
+    src.layout([0, 2, 3, 4, 6, 7, 8, 11, 14, 3])
+    src.observations(x, y, los, np.zeros(N), oid)
     t = torch.from_numpy(np.ascontiguousarray(theta, dtype=np.float64)).cuda()
     return src.forward(t, "fp32").cpu().numpy()

@@ -100,11 +100,18 @@ float from_bf16(__hip_bfloat16 x) { return __bfloat162float(x); }
 
 using namespace altar;
 
+// one diagonal block of a block-diagonal whitener: observations [off, off + n)
+// (off a multiple of 128), lower-triangular W of n x n stored npad x npad
+struct WBlock {
+    int off = 0, n = 0, npad = 0;
+    DevBuf W32, Whi, Wlo;
+};
+
 struct altar_source {
     int model = 0, P = 0, N = 0;
     double nu = 0.25;
     double fp64_sin = 0.4;  // CDM sides with |sin beta| below this run in FP64
-    int layout[8] = {0};
+    int layout[10] = {0};
     int nsets = 0;
     bool have_layout = false, have_obs = false, have_green = false, prewhitened = false;
     size_t Npad = 0, Ppad = 0;
@@ -116,6 +123,7 @@ struct altar_source {
     int wkind = 0;
     double normalization = 0.0;
     DevBuf W32, Whi, Wlo, W64;
+    std::vector<WBlock> blocks;  // ALTAR_WHITEN_BLOCKS
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
@@ -168,7 +176,8 @@ int altar_source_create(altar_source **out, int model, int P, int N, double nu)
 {
     return guarded([&] {
         require(out != nullptr, "null handle pointer");
-        require(model == ALTAR_MOGI || model == ALTAR_CDM || model == ALTAR_LINEAR, "unknown model");
+        require(model == ALTAR_MOGI || model == ALTAR_CDM || model == ALTAR_LINEAR || model == ALTAR_JOINT,
+                "unknown model");
         require(P > 0 && N > 0, "P and N must be positive");
         altar_source *h = new altar_source();
         h->model = model;
@@ -191,6 +200,10 @@ int altar_source_destroy(altar_source *h)
                          &h->Rlo, &h->R64, &h->partial, &h->T32, &h->Thi, &h->Tlo, &h->prm, &h->g32,
                          &h->g64, &h->flags, &h->shifts};
         for (DevBuf *b : all) b->release();
+        h->census.release();
+        for (WBlock &b : h->blocks) {
+            b.W32.release(); b.Whi.release(); b.Wlo.release();
+        }
         delete h;
     });
 }
@@ -199,7 +212,7 @@ int altar_source_layout(altar_source *h, const int *idx, int nidx)
 {
     return guarded([&] {
         require(h && idx, "null argument");
-        int want = h->model == ALTAR_MOGI ? 5 : h->model == ALTAR_CDM ? 7 : 0;
+        int want = h->model == ALTAR_MOGI ? 5 : h->model == ALTAR_CDM ? 7 : h->model == ALTAR_JOINT ? 10 : 0;
         require(h->model != ALTAR_LINEAR, "the linear model has no layout");
         require(nidx == want, "layout: wrong number of indices");
         int nsets = idx[nidx - 1], off = idx[nidx - 2];
@@ -207,12 +220,17 @@ int altar_source_layout(altar_source *h, const int *idx, int nidx)
         require((off < 0) == (nsets == 0), "layout: offsetIdx < 0 exactly when nsets == 0");
         for (int i = 0; i < nidx - 1; ++i)
             require(idx[i] < h->P, "layout: index beyond the parameter count");
+        auto mogi_ok = [&](const int *q) { return q[0] >= 0 && q[0] + 1 < h->P && q[1] >= 0 && q[2] >= 0; };
+        auto cdm_ok = [&](const int *q) {
+            return q[0] >= 0 && q[0] + 1 < h->P && q[1] >= 0 && q[2] >= 0 && q[3] >= 0 && q[3] + 2 < h->P &&
+                   q[4] >= 0 && q[4] + 2 < h->P;
+        };
         if (h->model == ALTAR_MOGI) {
-            require(idx[0] >= 0 && idx[0] + 1 < h->P && idx[1] >= 0 && idx[2] >= 0, "layout: bad Mogi index");
+            require(mogi_ok(idx), "layout: bad Mogi index");
+        } else if (h->model == ALTAR_CDM) {
+            require(cdm_ok(idx), "layout: bad CDM index");
         } else {
-            require(idx[0] >= 0 && idx[0] + 1 < h->P && idx[1] >= 0 && idx[2] >= 0 && idx[3] >= 0 &&
-                        idx[3] + 2 < h->P && idx[4] >= 0 && idx[4] + 2 < h->P,
-                    "layout: bad CDM index");
+            require(mogi_ok(idx) && cdm_ok(idx + 3), "layout: bad joint index");
         }
         if (nsets > 0) require(off + nsets <= h->P, "layout: offsets beyond the parameter count");
         for (int i = 0; i < nidx; ++i) h->layout[i] = idx[i];
@@ -283,6 +301,10 @@ int altar_source_whitening(altar_source *h, int kind, const double *W, double no
         require(h && W, "null argument");
         require(kind == ALTAR_WHITEN_DIAG || kind == ALTAR_WHITEN_LOWER, "whitening: unknown kind");
         const size_t N = h->N, Np = h->Npad;
+        for (WBlock &B : h->blocks) {
+            B.W32.release(); B.Whi.release(); B.Wlo.release();
+        }
+        h->blocks.clear();
         h->wkind = kind;
         h->normalization = normalization;
         if (kind == ALTAR_WHITEN_DIAG) {
@@ -307,6 +329,56 @@ int altar_source_whitening(altar_source *h, int kind, const double *W, double no
         for (size_t i = 0; i < N; ++i)
             for (size_t j = 0; j <= i; ++j) w64[i * N + j] = W[i * N + j];
         upload(h->W64, w64);
+    });
+}
+
+int altar_source_whitening_blocks(altar_source *h, int nblocks, const int *offsets, const int *sizes,
+                                  const double *W, double normalization)
+{
+    return guarded([&] {
+        require(h && offsets && sizes && W && nblocks > 0, "null argument");
+        int end = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            require(offsets[b] % 128 == 0, "whitening blocks: offsets must be multiples of 128");
+            require(sizes[b] > 0 && offsets[b] >= end && offsets[b] + sizes[b] <= h->N,
+                    "whitening blocks: blocks must be ordered, disjoint and inside [0, N)");
+            end = offsets[b] + sizes[b];
+        }
+        std::vector<WBlock> blocks(nblocks);
+        auto release = [](std::vector<WBlock> &v) {
+            for (WBlock &B : v) {
+                B.W32.release(); B.Whi.release(); B.Wlo.release();
+            }
+        };
+        try {
+            size_t at = 0;
+            for (int b = 0; b < nblocks; ++b) {
+                WBlock &B = blocks[b];
+                B.off = offsets[b];
+                B.n = sizes[b];
+                B.npad = (int)round_up((size_t)B.n, 128);
+                const size_t n = B.n, np = B.npad;
+                std::vector<float> w(np * np, 0.0f);
+                std::vector<__hip_bfloat16> wh(np * np, to_bf16(0.0f)), wl(np * np, to_bf16(0.0f));
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t j = 0; j <= i; ++j) {
+                        float v = (float)W[at + i * n + j];
+                        w[i * np + j] = v;
+                        __hip_bfloat16 hi = to_bf16(v);
+                        wh[i * np + j] = hi;
+                        wl[i * np + j] = to_bf16(v - from_bf16(hi));
+                    }
+                upload(B.W32, w); upload(B.Whi, wh); upload(B.Wlo, wl);
+                at += n * n;
+            }
+        } catch (...) {
+            release(blocks);
+            throw;
+        }
+        release(h->blocks);
+        h->blocks = std::move(blocks);
+        h->wkind = ALTAR_WHITEN_BLOCKS;
+        h->normalization = normalization;
     });
 }
 
@@ -348,7 +420,8 @@ MogiArgs mogi_args(altar_source *h, const double *theta, int C, bool fwd)
 {
     MogiArgs a;
     a.theta = theta; a.C = C; a.P = h->P;
-    a.xIdx = h->layout[0]; a.dIdx = h->layout[1]; a.sIdx = h->layout[2]; a.offsetIdx = h->layout[3];
+    a.xIdx = h->layout[0]; a.dIdx = h->layout[1]; a.sIdx = h->layout[2];
+    a.offsetIdx = h->model == ALTAR_JOINT ? h->layout[8] : h->layout[3];
     a.nsets = h->nsets; a.nu = h->nu;
     h->prm.reserve((size_t)h->Ccap * MOGI_PRM * sizeof(double));
     a.prm = h->prm.as<double>();
@@ -357,12 +430,20 @@ MogiArgs mogi_args(altar_source *h, const double *theta, int C, bool fwd)
     return a;
 }
 
+// the CDM part of a CDM or joint layout; in a joint source the dataset
+// shifts belong to the Mogi part (applied once)
+CdmLayout cdm_layout(const altar_source *h)
+{
+    if (h->model == ALTAR_JOINT)
+        return CdmLayout{h->layout[3], h->layout[4], h->layout[5], h->layout[6], h->layout[7], -1, 0};
+    return CdmLayout{h->layout[0], h->layout[1], h->layout[2], h->layout[3], h->layout[4], h->layout[5], h->nsets};
+}
+
 CdmArgs cdm_args(altar_source *h, const double *theta, int C, bool fwd)
 {
     CdmArgs a;
     a.theta = theta; a.C = C; a.P = h->P;
-    a.L = CdmLayout{h->layout[0], h->layout[1], h->layout[2], h->layout[3], h->layout[4], h->layout[5],
-                    h->nsets};
+    a.L = cdm_layout(h);
     a.nu = h->nu;
     a.fp64_sin = h->fp64_sin;
     h->g32.reserve((size_t)h->Ccap * sizeof(cdm::Geom<float>));
@@ -387,6 +468,22 @@ void run_source(altar_source *h, const double *theta, int C, int mode, bool fp64
         a.out = out;
         Mark m(h, 0, st);
         hip_check(launch_mogi(a, mode, fp64 ? 1 : 0, st), "mogi kernel");
+    } else if (h->model == ALTAR_JOINT) {
+        // superposition: the Mogi kernel writes pred - shift - d, the CDM
+        // kernel adds its prediction
+        require(!fp64, "the joint source computes in FP32 (FP64 near-singular CDM sides); ALTAR_FP64 is not offered");
+        require(mode == MODE_RES_F32 || mode == MODE_RES_F64, "joint source: residual modes only");
+        MogiArgs m = mogi_args(h, theta, C, fwd);
+        m.out = out;
+        {
+            Mark mk(h, 2, st);
+            hip_check(launch_mogi(m, mode, 0, st), "mogi kernel");
+        }
+        CdmArgs a = cdm_args(h, theta, C, fwd);
+        a.out = out;
+        hip_check(launch_cdm_prepare(a, st), "cdm prepare");
+        Mark mk(h, 0, st);
+        hip_check(launch_cdm_main(a, mode == MODE_RES_F32 ? MODE_ACC_F32 : MODE_ACC_F64, st), "cdm kernel");
     } else {
         require(!fp64, "the CDM kernels compute in FP32 with FP64 near-singular sides; ALTAR_FP64 is not offered");
         CdmArgs a = cdm_args(h, theta, C, fwd);
@@ -454,10 +551,9 @@ int altar_source_verify(altar_source *h, const double *theta_dev, int C, int *ma
 {
     return guarded([&] {
         require(h && theta_dev && mask_dev && C > 0, "bad argument");
-        if (h->model != ALTAR_CDM) return;
+        if (h->model != ALTAR_CDM && h->model != ALTAR_JOINT) return;
         require(h->have_layout, "layout not set");
-        CdmLayout L{h->layout[0], h->layout[1], h->layout[2], h->layout[3], h->layout[4], h->layout[5], h->nsets};
-        hip_check(launch_cdm_verify(theta_dev, C, h->P, L, mask_dev, S(stream)), "cdm verify");
+        hip_check(launch_cdm_verify(theta_dev, C, h->P, cdm_layout(h), mask_dev, S(stream)), "cdm verify");
     });
 }
 
@@ -590,6 +686,8 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         if (h->wkind == ALTAR_WHITEN_DIAG) {
             if (h->model == ALTAR_LINEAR)
                 throw Failure(ALTAR_EINVAL, "linear llk with a diagonal whitener: pre-whiten G and d instead");
+            if (h->model == ALTAR_JOINT)
+                throw Failure(ALTAR_EINVAL, "joint llk with a diagonal whitener: pass it as a lower or block whitener");
             const int nt = (int)h->ntiles_obs();
             const bool fp64 = precision == ALTAR_FP64;
             h->partial.reserve(Cp * nt * sizeof(double));
@@ -601,9 +699,11 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
             return;
         }
 
-        // full lower-triangular whitening
+        // full lower-triangular whitening, or block-diagonal (one triangular
+        // GEMM per block over the block's columns of R)
+        const bool blocked = h->wkind == ALTAR_WHITEN_BLOCKS;
         if (precision == ALTAR_FP64) {
-            require(h->model == ALTAR_MOGI, "FP64 full whitening is offered for the Mogi source only");
+            require(h->model == ALTAR_MOGI && !blocked, "FP64 full whitening is offered for the Mogi source only");
             h->R64.reserve((size_t)C * h->N * sizeof(double));
             Out o{};
             o.ld = h->N;
@@ -614,10 +714,16 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
                       "whiten f64");
             return;
         }
-        h->partial.reserve(Cp * ntn * sizeof(float));
+        int ntiles = ntn;
+        if (blocked) {
+            require(!wr_out, "fields: a lower-triangular whitener is required");
+            ntiles = 0;
+            for (const WBlock &B : h->blocks) ntiles += B.npad / 128;
+        }
+        h->partial.reserve(Cp * ntiles * sizeof(float));
         GemmEpi ep{};
         ep.partial = h->partial.as<float>();
-        ep.ntiles = ntn;
+        ep.ntiles = ntiles;
         int epi = EPI_SUMSQ;
         if (wr_out) {
             // the whitened residuals are stored too; rows past C land in a
@@ -627,22 +733,20 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
             ep.ldo = ldw;
             epi = EPI_STORE_SUMSQ;
         }
-        if (precision == ALTAR_FP32) {
-            if (h->model != ALTAR_LINEAR) {
-                h->R32.reserve(Cp * Np * sizeof(float), true);
-                Out o{};
-                o.ld = Np;
-                o.r32 = h->R32.as<float>();
-                run_source(h, theta_dev, C, MODE_RES_F32, false, false, o, st);
-            }
-            Mark m(h, 1, st);
-            hip_check(launch_gemm_f32(h->R32.as<float>(), Np, h->W32.as<float>(), Np, (int)Mp, (int)Np, (int)Np, 1,
-                                      epi, ep, st),
-                      "whitening gemm");
-        } else {
+        const bool bf16 = precision != ALTAR_FP32;
+        // residual matrix: FP32 (then split for BF16 when the source writes FP32 only)
+        const bool via_f32 = !bf16 || h->model == ALTAR_LINEAR || h->model == ALTAR_JOINT;
+        if (via_f32 && h->model != ALTAR_LINEAR) {
+            h->R32.reserve(Cp * Np * sizeof(float), true);
+            Out o{};
+            o.ld = Np;
+            o.r32 = h->R32.as<float>();
+            run_source(h, theta_dev, C, MODE_RES_F32, false, false, o, st);
+        }
+        if (bf16) {
             h->Rhi.reserve(Cp * Np * sizeof(__hip_bfloat16), true);
             h->Rlo.reserve(Cp * Np * sizeof(__hip_bfloat16), true);
-            if (h->model == ALTAR_LINEAR) {
+            if (via_f32) {
                 size_t n = Mp * Np;
                 hipLaunchKernelGGL(split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                                    h->R32.as<float>(), n, h->Rhi.as<__hip_bfloat16>(), h->Rlo.as<__hip_bfloat16>());
@@ -654,12 +758,34 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
                 o.rlo = h->Rlo.as<__hip_bfloat16>();
                 run_source(h, theta_dev, C, MODE_RES_BF16, false, false, o, st);
             }
-            Mark m(h, 1, st);
-            hip_check(launch_gemm_bf16x3(h->Rhi.p, h->Rlo.p, Np, h->Whi.p, h->Wlo.p, Np, (int)Mp, (int)Np, (int)Np, 1,
-                                         precision == ALTAR_BF16, epi, ep, st),
-                      "whitening gemm");
         }
-        hip_check(launch_llk_reduce(h->partial.p, 0, ntn, C, h->normalization, llk_dev, st), "reduce");
+        // one triangular GEMM: columns [off, off + npad) of R against W (npad x npad)
+        auto whiten = [&](size_t off, int npad, const DevBuf &W32, const DevBuf &Whi, const DevBuf &Wlo, int tile0) {
+            GemmEpi e = ep;
+            e.partial = ep.partial + tile0;
+            if (!bf16)
+                hip_check(launch_gemm_f32(h->R32.as<float>() + off, Np, W32.as<float>(), (size_t)npad, (int)Mp, npad,
+                                          npad, 1, epi, e, st),
+                          "whitening gemm");
+            else
+                hip_check(launch_gemm_bf16x3(h->Rhi.as<__hip_bfloat16>() + off, h->Rlo.as<__hip_bfloat16>() + off, Np,
+                                             Whi.p, Wlo.p, (size_t)npad, (int)Mp, npad, npad, 1,
+                                             precision == ALTAR_BF16, epi, e, st),
+                          "whitening gemm");
+        };
+        {
+            Mark m(h, 1, st);
+            if (!blocked) {
+                whiten(0, (int)Np, h->W32, h->Whi, h->Wlo, 0);
+            } else {
+                int tile0 = 0;
+                for (const WBlock &B : h->blocks) {
+                    whiten((size_t)B.off, B.npad, B.W32, B.Whi, B.Wlo, tile0);
+                    tile0 += B.npad / 128;
+                }
+            }
+        }
+        hip_check(launch_llk_reduce(h->partial.p, 0, ntiles, C, h->normalization, llk_dev, st), "reduce");
     }
 }
 }  // namespace
@@ -943,7 +1069,7 @@ int altar_source_fp64_sides(altar_source *h, double threshold)
 {
     return guarded([&] {
         require(h != nullptr, "null handle");
-        require(h->model == ALTAR_CDM, "fp64 sides: a CDM source is required");
+        require(h->model == ALTAR_CDM || h->model == ALTAR_JOINT, "fp64 sides: a CDM or joint source is required");
         require(threshold >= 0.0 && threshold <= 2.0, "fp64 sides: threshold must lie in [0, 2]");
         h->fp64_sin = threshold;
     });
@@ -953,7 +1079,7 @@ int altar_source_side_census(altar_source *h, int C, long long *counts, void *st
 {
     return guarded([&] {
         require(h != nullptr && counts != nullptr && C > 0, "bad argument");
-        require(h->model == ALTAR_CDM, "side census: a CDM source is required");
+        require(h->model == ALTAR_CDM || h->model == ALTAR_JOINT, "side census: a CDM or joint source is required");
         require(h->flags.p != nullptr && (size_t)C <= h->flags.bytes / sizeof(unsigned),
                 "side census: no forward call has prepared this many chains");
         hipStream_t st = S(stream);

@@ -15,6 +15,9 @@ constexpr int MOGI_PRM = 4 + MAX_SETS;
 
 // forward-kernel epilogues
 enum { MODE_DIAG = 0, MODE_RES_F32 = 1, MODE_RES_F64 = 2, MODE_RES_BF16 = 3 };
+// CDM only: add the predicted LOS displacement (no shift, no data) to an
+// existing residual matrix, FP32 or FP64 (the second source of a joint model)
+enum { MODE_ACC_F32 = 4, MODE_ACC_F64 = 5 };
 // GEMM epilogues
 enum { EPI_STORE = 0, EPI_SUMSQ = 1, EPI_SUMSQ_BIAS = 2, EPI_STORE_SUMSQ = 3 };
 

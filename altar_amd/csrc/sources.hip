@@ -192,14 +192,22 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
             }
             float fe = ue + (float)due, fn = un + (float)dun, fv = uv + (float)duv;
             float u = fe * obs.lx[n] + fn * obs.ly[n] + fv * obs.lz[n];
-            const float *sh = shifts + (size_t)chain * MAX_SETS;
-            float shift = 0.0f;
-            const int id = obs.oid[n];
-            for (int q = 0; q < nsets; ++q) shift = (id == q) ? sh[q] : shift;
-            val = u - shift - obs.d[n];
+            if constexpr (MODE == MODE_ACC_F32 || MODE == MODE_ACC_F64) {
+                val = u;
+            } else {
+                const float *sh = shifts + (size_t)chain * MAX_SETS;
+                float shift = 0.0f;
+                const int id = obs.oid[n];
+                for (int q = 0; q < nsets; ++q) shift = (id == q) ? sh[q] : shift;
+                val = u - shift - obs.d[n];
+            }
         }
     }
-    if constexpr (MODE == MODE_DIAG) {
+    if constexpr (MODE == MODE_ACC_F32) {
+        if (n < obs.N) out.r32[(size_t)chain * out.ld + n] += val;
+    } else if constexpr (MODE == MODE_ACC_F64) {
+        if (n < obs.N) out.r64[(size_t)chain * out.ld + n] += (double)val;
+    } else if constexpr (MODE == MODE_DIAG) {
         float w = n < obs.N ? obs.wd[n] : 0.0f;
         float z = val * w;
         block_partial<float>(z * z, (float *)out.partial, chain, blockIdx.y, gridDim.y);
@@ -263,6 +271,8 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
     if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG);
     else if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32);
     else if (mode == MODE_RES_F64) CDM_LAUNCH(MODE_RES_F64);
+    else if (mode == MODE_ACC_F32) CDM_LAUNCH(MODE_ACC_F32);
+    else if (mode == MODE_ACC_F64) CDM_LAUNCH(MODE_ACC_F64);
     else CDM_LAUNCH(MODE_RES_BF16);
 #undef CDM_LAUNCH
     return hipGetLastError();

@@ -23,38 +23,49 @@ def is_diagonal(cd):
     return not np.any(cd - np.diag(np.diag(cd)))
 
 
-def normalization(cd):
-    """-(N ln 2 pi + ln det Cd) / 2 (Mogi.py:327-340; Linear.py:315-328)."""
+def normalization(cd, chol=None):
+    """-(N ln 2 pi + ln det Cd) / 2 (Mogi.py:327-340; Linear.py:315-328).
+    ``chol`` may pass a Cholesky factor of ``cd`` already at hand."""
     cd = np.asarray(cd, dtype=np.float64)
     if cd.ndim == 1:
         logdet = float(np.sum(np.log(cd)))
         N = cd.shape[0]
     else:
-        L = np.linalg.cholesky(cd)
+        L = np.linalg.cholesky(cd) if chol is None else chol
         logdet = 2.0 * float(np.sum(np.log(np.diag(L))))
         N = cd.shape[0]
     return -0.5 * (np.log(2.0 * np.pi) * N + logdet)
 
 
-def whitener(cd, mode="exact"):
+def whitener(cd, mode="exact", chol=None):
     """(kind, W): kind "diag" with W the N inverse standard deviations, or
-    "lower" with W an N x N lower-triangular FP64 matrix."""
+    "lower" with W an N x N lower-triangular FP64 matrix.  ``chol`` may pass
+    a Cholesky factor of ``cd`` already at hand (exact mode)."""
     cd = np.asarray(cd, dtype=np.float64)
     if is_diagonal(cd):
         var = cd if cd.ndim == 1 else np.diag(cd)
         return "diag", 1.0 / np.sqrt(var)
     if mode == "exact":
-        from scipy.linalg import solve_triangular
-        L = np.linalg.cholesky(cd)
-        W = solve_triangular(L, np.eye(L.shape[0]), lower=True)
+        from scipy.linalg import lapack
+        L = np.linalg.cholesky(cd) if chol is None else chol
+        W, info = lapack.dtrtri(L, lower=1)
+        if info != 0:
+            raise np.linalg.LinAlgError(f"triangular inverse failed (info {info})")
         return "lower", np.ascontiguousarray(np.tril(W))
     if mode == "reference":
         return "lower", np.ascontiguousarray(np.linalg.cholesky(np.linalg.inv(cd)))
     raise ValueError(f"unknown whitening mode {mode!r}")
 
 
-def exponential_covariance(xy, sill, rng_length):
-    """Cd_ij = sill exp(-|s_i - s_j| / range) over station positions xy (N x 2)."""
+def exponential_covariance(xy, sill, rng_length, nugget=0.0):
+    """Cd_ij = sill exp(-|s_i - s_j| / range) + nugget delta_ij over station
+    positions xy (N x 2)."""
+    from scipy.spatial.distance import cdist
     xy = np.asarray(xy, dtype=np.float64)
-    d = np.sqrt(((xy[:, None, :] - xy[None, :, :]) ** 2).sum(-1))
-    return sill * np.exp(-d / rng_length)
+    cd = cdist(xy, xy)
+    np.multiply(cd, -1.0 / rng_length, out=cd)
+    np.exp(cd, out=cd)
+    cd *= sill
+    if nugget:
+        cd[np.diag_indices_from(cd)] += nugget
+    return cd

@@ -53,6 +53,15 @@ class Source:
         code = {"diag": _lib.WHITEN_DIAG, "lower": _lib.WHITEN_LOWER}[kind]
         _lib.call("altar_source_whitening", self.handle, code, _lib.ptr(W), float(normalization))
 
+    def whitening_blocks(self, offsets, blocks, normalization):
+        """Block-diagonal whitener: block b is the lower-triangular ``blocks[b]``
+        over the observations starting at ``offsets[b]`` (a multiple of 128)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        sizes = np.ascontiguousarray([np.asarray(W).shape[0] for W in blocks], dtype=np.int32)
+        W = np.concatenate([np.ascontiguousarray(B, dtype=np.float64).ravel() for B in blocks])
+        _lib.call("altar_source_whitening_blocks", self.handle, len(blocks), _lib.ptr(off), _lib.ptr(sizes),
+                  _lib.ptr(W), float(normalization))
+
     def _rows(self, theta, precision, fn):
         theta = theta.contiguous()
         C = theta.shape[0]

@@ -62,9 +62,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chains", type=int, default=16384)
     ap.add_argument("--obs", type=int, default=5000)
-    ap.add_argument("--workload", default="cdm", choices=["cdm", "linear", "mogi"],
+    ap.add_argument("--workload", default="cdm", choices=["cdm", "linear", "mogi", "joint"],
                     help="cdm: the headline case; linear: 4,000 x 400 static slip, full Cd; "
-                         "mogi: 1,000-point point source, diagonal Cd, annealed to beta = 1")
+                         "mogi: 1,000-point point source, diagonal Cd, annealed to beta = 1; "
+                         "joint: Mogi + CDM, 2 x 20,000 InSAR + 600 GNSS, block-diagonal Cd, 65,536 chains")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
     ap.add_argument("--fp64-sides", type=float, default=0.4,
                     help="CDM sides with |sin(beta)| below this are evaluated in FP64")
@@ -257,6 +258,18 @@ def side_workload(args):
         N, P = case["G"].shape
         flops_per_eval = 2 * N * P
         name = f"linear static slip, {C:,} chains x {N:,} obs x {P} params, full Cd (pre-whitened GEMM)"
+    elif args.workload == "joint":
+        from altar_amd.joint import Joint
+        C = 65536 if args.chains == 16384 else args.chains
+        case = cases.joint_case()
+        job = Job(chains=C, steps=1, seed=777 + 4)
+        model = Joint(case["psets"], case["datasets"], precision=args.precision, job=job,
+                      fp64_sides=args.fp64_sides)
+        N, P = sum(len(d.x) for d in case["datasets"]), 17
+        # lower-triangular whitening of each block: n_b (n_b + 1) flops per chain
+        flops_per_eval = sum(len(d.x) * (len(d.x) + 1) for d in case["datasets"])
+        name = (f"joint Mogi + CDM, {C:,} chains x {N:,} obs (2 InSAR tracks + GNSS), 17 params, "
+                f"block-diagonal Cd")
     else:
         from altar_amd.mogi import Mogi
         C = 1024
@@ -309,7 +322,18 @@ def side_workload(args):
         out["kernels"] = {"gemm": {"avg_ms": g, "tflops": C * flops_per_eval / (g * 1e-3) / 1e12
                                    if flops_per_eval else None}}
     if cnt[0]:
-        out.setdefault("kernels", {})["forward"] = {"avg_ms": ms[0] / cnt[0]}
+        f = ms[0] / cnt[0]
+        out.setdefault("kernels", {})["forward"] = {"avg_ms": f}
+        if args.workload == "joint":
+            pairs = C * model.observations
+            out["kernels"]["forward"]["cdm_tflops"] = pairs * CDM_FLOPS_PER_PAIR / (f * 1e-3) / 1e12
+    if cnt[2]:
+        out.setdefault("kernels", {})["mogi_forward"] = {"avg_ms": ms[2] / cnt[2]}
+    if args.workload == "joint":
+        f32s, f64s, skipped, invalid = src.side_census(C)
+        valid = max(C - invalid, 1)
+        out["cdm_sides"] = {"fp64_threshold": model.fp64_sides, "fp32_per_chain": f32s / valid,
+                            "fp64_per_chain": f64s / valid, "invalid_chains": invalid}
     if args.workload == "mogi":
         # the whole anneal from a fresh sample
         ann2 = Annealer(Metropolis(steps=5), COV(), seed=job.seed).initialize(job)

@@ -44,10 +44,12 @@ extern "C" {
 #define ALTAR_MOGI 0
 #define ALTAR_CDM 1
 #define ALTAR_LINEAR 2
+#define ALTAR_JOINT 3   /* Mogi + CDM sources superposed (no reference counterpart) */
 
 /* whitening operators (|W r|^2 with W lower triangular) */
 #define ALTAR_WHITEN_DIAG 1
 #define ALTAR_WHITEN_LOWER 2
+#define ALTAR_WHITEN_BLOCKS 3   /* block-diagonal, one lower-triangular block per dataset */
 
 /* arithmetic of the forward model + whitening */
 #define ALTAR_FP64 0          /* FP64 everywhere (small problems, error studies) */
@@ -63,8 +65,11 @@ int altar_version(void);
 
 /* ---- sources: one handle per forward model (replaces newSource, ext/mogi/source.cc:44-63) */
 
-/* model = ALTAR_MOGI | ALTAR_CDM | ALTAR_LINEAR; P parameters, N observations,
- * nu the Poisson ratio (ignored for ALTAR_LINEAR) */
+/* model = ALTAR_MOGI | ALTAR_CDM | ALTAR_LINEAR | ALTAR_JOINT; P parameters,
+ * N observations, nu the Poisson ratio (ignored for ALTAR_LINEAR).
+ * ALTAR_JOINT has no working reference counterpart (altar/altar/models/
+ * Ensemble.py:17-98 cannot run): the prediction is the sum of a Mogi source
+ * and a CDM source at every observation, minus the dataset shift. */
 int altar_source_create(altar_source **out, int model, int P, int N, double nu);
 int altar_source_destroy(altar_source *h);
 
@@ -72,6 +77,8 @@ int altar_source_destroy(altar_source *h);
  * ext/cdm/source.cc:373 "O!kkkkkk"):
  *   Mogi: idx = {xIdx, dIdx, sIdx, offsetIdx, nsets}           (y = xIdx + 1; s = log10 dV)
  *   CDM:  idx = {xIdx, dIdx, openingIdx, aXIdx, omegaXIdx, offsetIdx, nsets}
+ *   joint: idx = {Mogi xIdx, dIdx, sIdx, CDM xIdx, dIdx, openingIdx, aXIdx, omegaXIdx,
+ *                 offsetIdx, nsets}
  * offsetIdx = -1 with nsets = 0 means no per-dataset shifts; nsets <= 8. */
 int altar_source_layout(altar_source *h, const int *idx, int nidx);
 
@@ -94,6 +101,16 @@ int altar_source_green(altar_source *h, const double *G, const double *d, int pr
  * N x N row-major lower triangular (entries above the diagonal ignored).
  * llk = normalization - |W r|^2 / 2. */
 int altar_source_whitening(altar_source *h, int kind, const double *W, double normalization);
+
+/* Block-diagonal whitening (replaces a per-dataset cd_inv; the reference
+ * whitens one dataset per model, Mogi.py:327-356): block b covers the
+ * observations [offsets[b], offsets[b] + sizes[b]), offsets multiples of 128,
+ * blocks ordered and disjoint; W holds the blocks' lower-triangular
+ * whiteners one after the other, each sizes[b] x sizes[b] row-major.
+ * Observations outside every block do not enter the llk (padding).
+ * llk = normalization - sum_b |W_b r_b|^2 / 2.  FP32 and BF16 precisions. */
+int altar_source_whitening_blocks(altar_source *h, int nblocks, const int *offsets, const int *sizes,
+                                  const double *W, double normalization);
 
 /* forwardModelBatched: predicted LOS displacements minus the dataset shift
  * (Source::displacements, libmogi/Source.cc:65-118), FP64 out, C x ldo rows.
