@@ -17,7 +17,7 @@ from .source import HIP, Source, set_covariance
 
 class CDM(Bayesian):
     def __init__(self, psets, x, y, los, d, oid, cd, nu=0.25, whiten="exact", precision="fp32",
-                 job=None, device="cuda", fp64_sides=0.4):
+                 job=None, device="cuda", fp64_sides=0.2):
         super().__init__(psets, job=job, device=device)
         self.x, self.y = np.asarray(x, float), np.asarray(y, float)
         self.los, self.d = np.asarray(los, float), np.asarray(d, float)

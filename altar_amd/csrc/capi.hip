@@ -110,7 +110,7 @@ struct WBlock {
 struct altar_source {
     int model = 0, P = 0, N = 0;
     double nu = 0.25;
-    double fp64_sin = 0.4;  // CDM sides with |sin beta| below this run in FP64
+    double fp64_sin = 0.2;  // CDM sides with |sin beta| below this run in FP64
     int layout[10] = {0};
     int nsets = 0;
     bool have_layout = false, have_obs = false, have_green = false, prewhitened = false;
@@ -127,7 +127,7 @@ struct altar_source {
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
-    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census;
+    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census, fp64_list, fp64_count;
     // optional per-kernel timing with HIP events on the launch stream
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[ALTAR_TIMER_SLOTS];
@@ -156,6 +156,7 @@ struct altar_source {
         // zero-initialised: the GEMMs read the padding columns
         R32.release(); Rhi.release(); Rlo.release(); R64.release(); T32.release(); Thi.release(); Tlo.release();
         partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release(); census.release();
+        fp64_list.release(); fp64_count.release();
     }
     size_t ntiles_obs() const { return (N + 255) / 256; }
 };
@@ -201,6 +202,8 @@ int altar_source_destroy(altar_source *h)
                          &h->g64, &h->flags, &h->shifts};
         for (DevBuf *b : all) b->release();
         h->census.release();
+        h->fp64_list.release();
+        h->fp64_count.release();
         for (WBlock &b : h->blocks) {
             b.W32.release(); b.Whi.release(); b.Wlo.release();
         }
@@ -454,6 +457,10 @@ CdmArgs cdm_args(altar_source *h, const double *theta, int C, bool fwd)
     a.g64 = h->g64.as<cdm::Geom<double>>();
     a.flags = h->flags.as<unsigned>();
     a.shifts = h->shifts.as<float>();
+    h->fp64_list.reserve((size_t)h->Ccap * sizeof(int));
+    h->fp64_count.reserve(sizeof(int));
+    a.fp64_list = h->fp64_list.as<int>();
+    a.fp64_count = h->fp64_count.as<int>();
     a.obs = h->obs(fwd);
     a.out = Out{};
     return a;
@@ -735,7 +742,7 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         }
         const bool bf16 = precision != ALTAR_FP32;
         // residual matrix: FP32 (then split for BF16 when the source writes FP32 only)
-        const bool via_f32 = !bf16 || h->model == ALTAR_LINEAR || h->model == ALTAR_JOINT;
+        const bool via_f32 = !bf16 || h->model != ALTAR_MOGI;
         if (via_f32 && h->model != ALTAR_LINEAR) {
             h->R32.reserve(Cp * Np * sizeof(float), true);
             Out o{};

@@ -17,6 +17,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 namespace altar {
 
 constexpr int OBS_TILE = 256;
@@ -136,7 +138,10 @@ __global__ void cdm_prepare_kernel(const double *__restrict__ theta, int C, int 
             (L.offsetIdx >= 0 && k < L.nsets) ? (float)t[L.offsetIdx + k] : 0.0f;
 }
 
-template <int MODE>
+// INLINE64: the FP64 sides are evaluated here; otherwise they are skipped and
+// cdm_fp64_kernel adds them afterwards (residual modes), which keeps this
+// kernel at about half the registers and twice the waves per SIMD.
+template <int MODE, bool INLINE64>
 __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::Geom<float> *__restrict__ g32,
                                                        const cdm::Geom<double> *__restrict__ g64,
                                                        const unsigned *__restrict__ flags,
@@ -153,41 +158,52 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
             val = __builtin_nanf("");
         } else {
             const float X = obs.x[n], Y = obs.y[n];
-            float dx[12], dy[12], r[12], ir[12], irpa[12];
-#pragma unroll
-            for (int v = 0; v < 12; ++v) {
-                dx[v] = X - G.vx[v];
-                dy[v] = Y - G.vy[v];
-                float a = G.va[v];
-                r[v] = cdm::f_sqrt(dx[v] * dx[v] + dy[v] * dy[v] + a * a);
-                ir[v] = cdm::f_rcp(r[v]);
-                irpa[v] = cdm::f_rcp(r[v] + a);
-            }
             const float k = 1.0f - 2.0f * nu;
             float ue = 0.0f, un = 0.0f, uv = 0.0f;
             double due = 0.0, dun = 0.0, duv = 0.0;
+            // rectangle by rectangle: the four sides of rectangle q join its
+            // vertices 4q .. 4q + 3 only, so the per-vertex terms live for one
+            // rectangle (fewer registers, more waves per SIMD)
 #pragma unroll
-            for (int s = 0; s < 12; ++s) {
-                if (fl & cdm::skip_bit(s)) continue;
-                const int a = cdm::side_vA(s), b = cdm::side_vB(s);
-                if (fl & cdm::fp64_bit(s)) {
-                    // near-vertical side: the pair is evaluated in FP64
-                    const cdm::Geom<double> &H = g64[chain];
-                    double Xd = obs.xd[n], Yd = obs.yd[n];
-                    double dxa = Xd - H.vx[a], dya = Yd - H.vy[a], aa = H.va[a];
-                    double dxb = Xd - H.vx[b], dyb = Yd - H.vy[b], ab = H.va[b];
-                    double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
-                    double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
-                    double e, nn, u;
-                    cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya,
-                                           aa, H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa), ab, H.ka2[b],
-                                           rb, cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
-                    due += e; dun += nn; duv += u;
-                } else {
-                    float e, nn, u;
-                    cdm::side_disp<float>(G.side[s], nu, k, dx[a], dy[a], G.va[a], G.ka2[a], r[a], ir[a],
-                                          irpa[a], G.va[b], G.ka2[b], r[b], ir[b], irpa[b], e, nn, u);
-                    ue += e; un += nn; uv += u;
+            for (int q = 0; q < 3; ++q) {
+                if ((fl & (0xfu << (4 * q))) == (0xfu << (4 * q))) continue;  // rectangle switched off
+                float dx[4], dy[4], r[4], ir[4], irpa[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int v = 4 * q + j;
+                    dx[j] = X - G.vx[v];
+                    dy[j] = Y - G.vy[v];
+                    float a = G.va[v];
+                    r[j] = cdm::f_sqrt(dx[j] * dx[j] + dy[j] * dy[j] + a * a);
+                    ir[j] = cdm::f_rcp(r[j]);
+                    irpa[j] = cdm::f_rcp(r[j] + a);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int s = 4 * q + j;
+                    if (fl & cdm::skip_bit(s)) continue;
+                    const int a = cdm::side_vA(s), b = cdm::side_vB(s);
+                    const int ja = a - 4 * q, jb = b - 4 * q;
+                    if (!INLINE64 && (fl & cdm::fp64_bit(s))) continue;
+                    if (INLINE64 && (fl & cdm::fp64_bit(s))) {
+                        // near-vertical side: the pair is evaluated in FP64
+                        const cdm::Geom<double> &H = g64[chain];
+                        double Xd = obs.xd[n], Yd = obs.yd[n];
+                        double dxa = Xd - H.vx[a], dya = Yd - H.vy[a], aa = H.va[a];
+                        double dxb = Xd - H.vx[b], dyb = Yd - H.vy[b], ab = H.va[b];
+                        double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
+                        double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
+                        double e, nn, u;
+                        cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya, aa,
+                                               H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa), ab, H.ka2[b], rb,
+                                               cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
+                        due += e; dun += nn; duv += u;
+                    } else {
+                        float e, nn, u;
+                        cdm::side_disp<float>(G.side[s], nu, k, dx[ja], dy[ja], G.va[a], G.ka2[a], r[ja], ir[ja],
+                                              irpa[ja], G.va[b], G.ka2[b], r[jb], ir[jb], irpa[jb], e, nn, u);
+                        ue += e; un += nn; uv += u;
+                    }
                 }
             }
             float fe = ue + (float)due, fn = un + (float)dun, fv = uv + (float)duv;
@@ -263,18 +279,121 @@ hipError_t launch_cdm(const CdmArgs &a, int mode, hipStream_t st)
     return launch_cdm_main(a, mode, st);
 }
 
+// Chains that have FP64 sides (valid chains only), in chain order: one
+// workgroup, each thread a contiguous range, a fixed-order scan of the counts.
+constexpr int LIST_THREADS = 1024;
+__global__ __launch_bounds__(LIST_THREADS) void fp64_chain_list_kernel(const unsigned *__restrict__ flags, int C,
+                                                                       int *__restrict__ list,
+                                                                       int *__restrict__ count)
+{
+    __shared__ int part[LIST_THREADS];
+    const int per = (C + LIST_THREADS - 1) / LIST_THREADS;
+    const int b = threadIdx.x * per, e = min(C, b + per);
+    auto has64 = [&](int c) {
+        const unsigned f = flags[c];
+        return !(f & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) && (((f >> 12) & ~f & 0xfffu) != 0u);
+    };
+    int n = 0;
+    for (int c = b; c < e; ++c) n += has64(c) ? 1 : 0;
+    part[threadIdx.x] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < LIST_THREADS; ++t) {
+            const int v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        *count = run;
+    }
+    __syncthreads();
+    int at = part[threadIdx.x];
+    for (int c = b; c < e; ++c)
+        if (has64(c)) list[at++] = c;
+}
+
+// The FP64 sides of the listed chains, added to the residual matrix (FP32
+// or FP64).  Block (tile, g) walks the list from g in steps of gridDim.y;
+// the loop ends for every block at the list's length.
+template <int MODE>
+__global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const cdm::Geom<double> *__restrict__ g64,
+                                                            const unsigned *__restrict__ flags,
+                                                            const int *__restrict__ list,
+                                                            const int *__restrict__ count, double nu, Out out)
+{
+    const int n = blockIdx.x * OBS_TILE + threadIdx.x;
+    const int cnt = *count;
+    double Xd = 0.0, Yd = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+    if (n < obs.N) {
+        Xd = obs.xd[n]; Yd = obs.yd[n];
+        lx = obs.lxd[n]; ly = obs.lyd[n]; lz = obs.lzd[n];
+    }
+    const double k = 1.0 - 2.0 * nu;
+    for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
+        const int chain = list[i];
+        const unsigned fl = flags[chain];
+        const cdm::Geom<double> &H = g64[chain];
+        double due = 0.0, dun = 0.0, duv = 0.0;
+#pragma unroll
+        for (int s = 0; s < 12; ++s) {
+            if (!(fl & cdm::fp64_bit(s)) || (fl & cdm::skip_bit(s))) continue;
+            const int a = cdm::side_vA(s), b = cdm::side_vB(s);
+            double dxa = Xd - H.vx[a], dya = Yd - H.vy[a], aa = H.va[a];
+            double dxb = Xd - H.vx[b], dyb = Yd - H.vy[b], ab = H.va[b];
+            double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
+            double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
+            double e, nn, u;
+            cdm::side_disp<double>(H.side[s], nu, k, dxa, dya, aa, H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa),
+                                   ab, H.ka2[b], rb, cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
+            due += e; dun += nn; duv += u;
+        }
+        if (n < obs.N) {
+            const double u = due * lx + dun * ly + duv * lz;
+            if constexpr (MODE == MODE_ACC_F32)
+                out.r32[(size_t)chain * out.ld + n] += (float)u;
+            else
+                out.r64[(size_t)chain * out.ld + n] += u;
+        }
+    }
+}
+
 hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
 {
     dim3 grid(a.C, (a.obs.N + OBS_TILE - 1) / OBS_TILE), block(OBS_TILE);
     float nu = (float)a.nu;
-#define CDM_LAUNCH(M) hipLaunchKernelGGL((cdm_kernel<M>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, a.out)
-    if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG);
-    else if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32);
-    else if (mode == MODE_RES_F64) CDM_LAUNCH(MODE_RES_F64);
-    else if (mode == MODE_ACC_F32) CDM_LAUNCH(MODE_ACC_F32);
-    else if (mode == MODE_ACC_F64) CDM_LAUNCH(MODE_ACC_F64);
-    else CDM_LAUNCH(MODE_RES_BF16);
+    // residual modes take the two-pass path (FP32 sides, then the FP64 sides
+    // of the chains that have any); the partial-sum and BF16 epilogues need
+    // the complete residual in one kernel
+    const bool split = (mode == MODE_RES_F32 || mode == MODE_RES_F64 || mode == MODE_ACC_F32 ||
+                        mode == MODE_ACC_F64) && a.fp64_list != nullptr && a.fp64_count != nullptr;
+#define CDM_LAUNCH(M, I) hipLaunchKernelGGL((cdm_kernel<M, I>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, a.out)
+    if (!split) {
+        if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG, true);
+        else if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32, true);
+        else if (mode == MODE_RES_F64) CDM_LAUNCH(MODE_RES_F64, true);
+        else if (mode == MODE_ACC_F32) CDM_LAUNCH(MODE_ACC_F32, true);
+        else if (mode == MODE_ACC_F64) CDM_LAUNCH(MODE_ACC_F64, true);
+        else CDM_LAUNCH(MODE_RES_BF16, true);
+        return hipGetLastError();
+    }
+    if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32, false);
+    else if (mode == MODE_RES_F64) CDM_LAUNCH(MODE_RES_F64, false);
+    else if (mode == MODE_ACC_F32) CDM_LAUNCH(MODE_ACC_F32, false);
+    else CDM_LAUNCH(MODE_ACC_F64, false);
 #undef CDM_LAUNCH
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fp64_chain_list_kernel, dim3(1), dim3(LIST_THREADS), 0, st, a.flags, a.C, a.fp64_list,
+                       a.fp64_count);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 g2((a.obs.N + OBS_TILE - 1) / OBS_TILE, (unsigned)std::min(a.C, 2048));
+    if (mode == MODE_RES_F32 || mode == MODE_ACC_F32)
+        hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F32>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
+                           a.fp64_count, a.nu, a.out);
+    else
+        hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F64>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
+                           a.fp64_count, a.nu, a.out);
     return hipGetLastError();
 }
 

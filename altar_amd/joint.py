@@ -74,7 +74,7 @@ def layout_datasets(datasets):
 
 class Joint(Bayesian):
     def __init__(self, psets, datasets, nu=0.25, whiten="exact", precision="fp32", job=None, device="cuda",
-                 fp64_sides=0.4):
+                 fp64_sides=0.2):
         super().__init__(psets, job=job, device=device)
         self.datasets = list(datasets)
         self.nu = nu

@@ -67,7 +67,7 @@ def parse():
                          "mogi: 1,000-point point source, diagonal Cd, annealed to beta = 1; "
                          "joint: Mogi + CDM, 2 x 20,000 InSAR + 600 GNSS, block-diagonal Cd, 65,536 chains")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16r", "bf16"])
-    ap.add_argument("--fp64-sides", type=float, default=0.4,
+    ap.add_argument("--fp64-sides", type=float, default=0.2,
                     help="CDM sides with |sin(beta)| below this are evaluated in FP64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -143,7 +143,7 @@ int altar_source_verify(altar_source *h, const double *theta_dev, int C, int *ma
 
 /* CDM precision policy (no reference counterpart: the reference evaluates
  * every side in FP64, models/cdm/cdm/libcdm.py:375-430).  Sides whose
- * |sin(beta)| lies below threshold (default 0.4) are evaluated in FP64, the
+ * |sin(beta)| lies below threshold (default 0.2) are evaluated in FP64, the
  * rest in FP32; 0 turns the FP64 sides off and any value above 1 (at most
  * 2) evaluates every side in FP64. */
 int altar_source_fp64_sides(altar_source *h, double threshold);

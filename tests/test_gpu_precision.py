@@ -65,10 +65,17 @@ def test_precision_table(cuda, oracle):
     model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
                 job=Job(chains=C)).initialize()
     t = torch.from_numpy(theta).to(cuda)
-    got = model.forwardModelBatched(t, "fp32").cpu().numpy()
-    fe = np.abs(got - pred).max(axis=1) / np.abs(pred).max(axis=1)
-    out["cdm_forward_fp32_normwise"] = stats(fe)
-    assert fe.max() < 2e-4
+    for thr in (0.4, 0.3, 0.2, 0.1):
+        model.source.fp64_sides(thr)
+        got = model.forwardModelBatched(t, "fp32").cpu().numpy()
+        fe = np.abs(got - pred).max(axis=1) / np.abs(pred).max(axis=1)
+        out[f"cdm_forward_fp32_normwise_fp64sides{thr}"] = stats(fe)
+        llk = torch.empty(C, dtype=torch.float64, device=cuda)
+        model.source.data_llk(t, llk, "fp32")
+        d = np.abs(llk.cpu().numpy() - ref)
+        out[f"cdm_llk_fp32_fp64sides{thr}"] = {"abs": stats(d), "abs_near_truth": stats(d[64:])}
+        assert fe.max() < 2e-4
+    model.source.fp64_sides(model.fp64_sides)
     for prec, tol in (("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
         llk = torch.empty(C, dtype=torch.float64, device=cuda)
         model.source.data_llk(t, llk, prec)
