@@ -118,7 +118,7 @@ struct altar_source {
     // observations
     DevBuf x, y, lx, ly, lz, d, wd, xd, yd, lxd, lyd, lzd, dd, wdd, oid, zf, zd;
     // linear operator: G (Npad x Ppad FP32 and split BF16), d (Npad FP32)
-    DevBuf G32, Ghi, Glo, dlin;
+    DevBuf G32, Ghi, Glo, G64, cbias, cref;
     // whitening
     int wkind = 0;
     double normalization = 0.0;
@@ -196,8 +196,8 @@ int altar_source_destroy(altar_source *h)
     return guarded([&] {
         if (!h) return;
         DevBuf *all[] = {&h->x, &h->y, &h->lx, &h->ly, &h->lz, &h->d, &h->wd, &h->xd, &h->yd, &h->lxd,
-                         &h->lyd, &h->lzd, &h->dd, &h->wdd, &h->oid, &h->zf, &h->zd, &h->G32, &h->Ghi,
-                         &h->Glo, &h->dlin, &h->W32, &h->Whi, &h->Wlo, &h->W64, &h->R32, &h->Rhi,
+                         &h->lyd, &h->lzd, &h->dd, &h->wdd, &h->oid, &h->zf, &h->zd, &h->G64, &h->cbias, &h->cref, &h->G32, &h->Ghi,
+                         &h->Glo, &h->W32, &h->Whi, &h->Wlo, &h->W64, &h->R32, &h->Rhi,
                          &h->Rlo, &h->R64, &h->partial, &h->T32, &h->Thi, &h->Tlo, &h->prm, &h->g32,
                          &h->g64, &h->flags, &h->shifts};
         for (DevBuf *b : all) b->release();
@@ -277,7 +277,7 @@ int altar_source_green(altar_source *h, const double *G, const double *d, int pr
         require(h && G && d, "null argument");
         require(h->model == ALTAR_LINEAR, "green: not a linear source");
         const size_t N = h->N, P = h->P, Np = h->Npad, Pp = h->Ppad;
-        std::vector<float> g(Np * Pp, 0.0f), dl(Np, 0.0f);
+        std::vector<float> g(Np * Pp, 0.0f);
         std::vector<__hip_bfloat16> gh(Np * Pp, to_bf16(0.0f)), gl(Np * Pp, to_bf16(0.0f));
         for (size_t n = 0; n < N; ++n) {
             for (size_t p = 0; p < P; ++p) {
@@ -287,10 +287,11 @@ int altar_source_green(altar_source *h, const double *G, const double *d, int pr
                 gh[n * Pp + p] = hi;
                 gl[n * Pp + p] = to_bf16(v - from_bf16(hi));
             }
-            dl[n] = (float)d[n];
         }
-        upload(h->G32, g); upload(h->Ghi, gh); upload(h->Glo, gl); upload(h->dlin, dl);
-        // FP64 copies for the reference-precision path
+        upload(h->G32, g); upload(h->Ghi, gh); upload(h->Glo, gl);
+        // FP64 copies: the centred residual (linear_center) and the
+        // reference-precision path
+        upload(h->G64, std::vector<double>(G, G + N * P));
         upload(h->dd, std::vector<double>(d, d + N));
         h->prewhitened = prewhitened != 0;
         h->have_green = true;
@@ -501,15 +502,48 @@ void run_source(altar_source *h, const double *theta, int C, int mode, bool fp64
     }
 }
 
-// theta (C x P FP64) -> padded FP32 / split BF16 operand for the linear GEMMs
+// The linear residual is formed about a reference sample theta_ref (the
+// mean of the call's first 64 chains): G theta - d = G (theta - theta_ref) - b with
+// b = d - G theta_ref in FP64.  The GEMM then multiplies small offsets, and
+// the cancellation between G theta and d (whitened data are tens of times
+// the residual) happens in FP64 instead of in the FP32/BF16 product.
+// theta_ref[p] = mean of theta[0 .. min(C, 64) - 1][p], in chain order
+constexpr int REF_ROWS = 64;
+__global__ void theta_ref_kernel(const double *__restrict__ theta, int C, int P, double *__restrict__ ref)
+{
+    const int n = min(C, REF_ROWS);
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        double s = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < n; ++c) s += theta[(size_t)c * P + p];
+        ref[p] = s / n;
+    }
+}
+
+// b[n] for n < N, one wavefront per observation; zero past N.
+__global__ void center_bias_kernel(const double *__restrict__ G, const double *__restrict__ d,
+                                   const double *__restrict__ theta_ref, int N, int Np, int P,
+                                   float *__restrict__ b)
+{
+    const int n = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (n >= Np) return;
+    double acc = 0.0;
+    if (n < N)
+        for (int p = lane; p < P; p += 64) acc += G[(size_t)n * P + p] * theta_ref[p];
+    acc = wave_sum(acc);
+    if (lane == 0) b[n] = n < N ? (float)(d[n] - acc) : 0.0f;
+}
+
+// theta (C x P FP64) -> padded FP32 / split BF16 operand for the linear
+// GEMMs, as offsets from theta_ref
 __global__ void theta_pack_kernel(const double *__restrict__ theta, int C, int P, size_t ld,
-                                  float *__restrict__ t32, __hip_bfloat16 *__restrict__ thi,
-                                  __hip_bfloat16 *__restrict__ tlo)
+                                  const double *__restrict__ theta_ref, float *__restrict__ t32,
+                                  __hip_bfloat16 *__restrict__ thi, __hip_bfloat16 *__restrict__ tlo)
 {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)C * P) return;
     int c = (int)(idx / P), p = (int)(idx % P);
-    float v = (float)theta[idx];
+    float v = (float)(theta[idx] - theta_ref[p]);
     t32[(size_t)c * ld + p] = v;
     __hip_bfloat16 hi = __float2bfloat16(v);
     thi[(size_t)c * ld + p] = hi;
@@ -528,20 +562,54 @@ __global__ void split_kernel(const float *__restrict__ r, size_t n, __hip_bfloat
     lo[i] = __float2bfloat16(v - __bfloat162float(h));
 }
 
+// b = d - G theta_ref (centred residual) into h->cbias, and the packed
+// offsets theta - theta_ref
+void linear_center(altar_source *h, const double *theta, int C, hipStream_t st)
+{
+    const size_t Pp = h->Ppad, Np = h->Npad, Cp = h->Cpad;
+    h->cref.reserve((size_t)h->P * sizeof(double));
+    hipLaunchKernelGGL(theta_ref_kernel, dim3(1), dim3(256), 0, st, theta, C, h->P, h->cref.as<double>());
+    hip_check(hipGetLastError(), "centre");
+    const double *ref = h->cref.as<double>();
+    h->cbias.reserve(Np * sizeof(float), true);
+    hipLaunchKernelGGL(center_bias_kernel, dim3((unsigned)((Np * 64 + 255) / 256)), dim3(256), 0, st,
+                       h->G64.as<double>(), h->dd.as<double>(), ref, h->N, (int)Np, h->P, h->cbias.as<float>());
+    hip_check(hipGetLastError(), "centre bias");
+    h->T32.reserve(Cp * Pp * sizeof(float), true);
+    h->Thi.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
+    h->Tlo.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
+    hipLaunchKernelGGL(theta_pack_kernel, dim3((unsigned)(((long)C * h->P + 255) / 256)), dim3(256), 0, st, theta, C,
+                       h->P, Pp, ref, h->T32.as<float>(), h->Thi.as<__hip_bfloat16>(), h->Tlo.as<__hip_bfloat16>());
+    hip_check(hipGetLastError(), "theta pack");
+}
+
+// theta (C x P FP64) -> padded FP32 operand without centring (forward)
+__global__ void theta_pack_plain_kernel(const double *__restrict__ theta, int C, int P, size_t ld,
+                                        float *__restrict__ t32)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)C * P) return;
+    int c = (int)(idx / P), p = (int)(idx % P);
+    t32[(size_t)c * ld + p] = (float)theta[idx];
+}
+
 void linear_residual_f32(altar_source *h, const double *theta, int C, bool fwd, hipStream_t st)
 {
     // buffers are sized for the handle's capacity; the launches cover the
     // rows of this call only (C rounded up to the 128-row tile)
     const size_t Pp = h->Ppad, Np = h->Npad, Cp = h->Cpad, Mp = round_up((size_t)C, 128);
-    h->T32.reserve(Cp * Pp * sizeof(float), true);
-    h->Thi.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
-    h->Tlo.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
-    hipLaunchKernelGGL(theta_pack_kernel, dim3((unsigned)(((long)C * h->P + 255) / 256)), dim3(256), 0, st, theta, C,
-                       h->P, Pp, h->T32.as<float>(), h->Thi.as<__hip_bfloat16>(), h->Tlo.as<__hip_bfloat16>());
-    hip_check(hipGetLastError(), "theta pack");
+    if (fwd) {
+        // predictions G theta: no centring
+        h->T32.reserve(Cp * Pp * sizeof(float), true);
+        hipLaunchKernelGGL(theta_pack_plain_kernel, dim3((unsigned)(((long)C * h->P + 255) / 256)), dim3(256), 0,
+                           st, theta, C, h->P, Pp, h->T32.as<float>());
+        hip_check(hipGetLastError(), "theta pack");
+    } else {
+        linear_center(h, theta, C, st);
+    }
     h->R32.reserve(Cp * Np * sizeof(float), true);
     GemmEpi ep{};
-    ep.bias = fwd ? nullptr : h->dlin.as<float>();
+    ep.bias = fwd ? nullptr : h->cbias.as<float>();
     ep.out = h->R32.as<float>();
     ep.ldo = Np;
     Mark m(h, 0, st);
@@ -661,18 +729,12 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         if (h->model == ALTAR_LINEAR) {
             require(precision != ALTAR_FP64, "linear llk: FP64 is not offered");
             if (h->prewhitened) {
-                // |G' theta - d'|^2 in one GEMM with the bias fused in the epilogue
+                // |G' theta - d'|^2 in one GEMM with the (centred) bias fused in the epilogue
                 const size_t Pp = h->Ppad;
-                h->T32.reserve(Cp * Pp * sizeof(float), true);
-                h->Thi.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
-                h->Tlo.reserve(Cp * Pp * sizeof(__hip_bfloat16), true);
-                hipLaunchKernelGGL(theta_pack_kernel, dim3((unsigned)(((long)C * h->P + 255) / 256)), dim3(256), 0,
-                                   st, theta_dev, C, h->P, Pp, h->T32.as<float>(), h->Thi.as<__hip_bfloat16>(),
-                                   h->Tlo.as<__hip_bfloat16>());
-                hip_check(hipGetLastError(), "theta pack");
+                linear_center(h, theta_dev, C, st);
                 h->partial.reserve(Cp * ntn * sizeof(float));
                 GemmEpi ep{};
-                ep.bias = h->dlin.as<float>();
+                ep.bias = h->cbias.as<float>();
                 ep.partial = h->partial.as<float>();
                 ep.ntiles = ntn;
                 Mark m(h, 1, st);
