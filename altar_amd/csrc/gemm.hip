@@ -121,9 +121,15 @@ __device__ inline void tile_coords(int tri, int ntn, int &tn, int &tm)
     tn = tri ? ntn - 1 - cls : cls;
 }
 
+// LDS rows hold one K slice of 4 chunks of 16 bytes.  Chunk c of row r is
+// stored at chunk position c ^ swz(r), swz(r) = (r / 4) % 4, so that both
+// the staging writes (4 lanes per row, 4 rows per 16 lanes) and the fragment
+// reads (16 rows, one chunk each) hit 16 distinct 4-bank groups.
+__device__ inline int swz(int r) { return (r >> 2) & 3; }
+
 // ---------------------------------------------------------------- FP32
 constexpr int BK32 = 16;
-constexpr int LDS32 = BK32 + 4;  // row pitch (floats): conflict-free 16-byte reads
+constexpr int LDS32 = BK32;  // row pitch (floats), unpadded: the chunk swizzle removes bank conflicts
 
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, size_t lda,
@@ -143,8 +149,9 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
     const int nk = kend / BK32;
 
     // global -> register staging: thread t moves rows t/4 and t/4 + 64,
-    // columns 4 (t % 4) .. +3 of each operand tile
+    // columns 4 (t % 4) .. +3 of each operand tile (LDS chunk (t % 4) ^ swz)
     const int lr = tid >> 2, lc = (tid & 3) * 4;
+    const int ls = 4 * ((tid & 3) ^ swz(lr));  // swz(lr + 64) == swz(lr)
     const float *Ag0 = A + (size_t)(m0 + lr) * lda + lc;
     const float *Ag1 = A + (size_t)(m0 + lr + 64) * lda + lc;
     const float *Bg0 = B + (size_t)(n0 + lr) * ldb + lc;
@@ -159,10 +166,10 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
         rb1 = *(const floatx4 *)(Bg1 + k);
     };
     auto sstore = [&](int buf) {
-        *(floatx4 *)(&As[buf][lr * LDS32 + lc]) = ra0;
-        *(floatx4 *)(&As[buf][(lr + 64) * LDS32 + lc]) = ra1;
-        *(floatx4 *)(&Bs[buf][lr * LDS32 + lc]) = rb0;
-        *(floatx4 *)(&Bs[buf][(lr + 64) * LDS32 + lc]) = rb1;
+        *(floatx4 *)(&As[buf][lr * LDS32 + ls]) = ra0;
+        *(floatx4 *)(&As[buf][(lr + 64) * LDS32 + ls]) = ra1;
+        *(floatx4 *)(&Bs[buf][lr * LDS32 + ls]) = rb0;
+        *(floatx4 *)(&Bs[buf][(lr + 64) * LDS32 + ls]) = rb1;
     };
 
     floatx16 acc[2][2];
@@ -174,6 +181,8 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.0f;
 
     const int half = lane >> 5, l32 = lane & 31;
+    // this lane's rows are 32-aligned bases + l32: the swizzle depends on l32 only
+    const int c0 = 4 * ((2 * half) ^ swz(l32)), c1 = 4 * ((2 * half + 1) ^ swz(l32));
     if (nk > 0) {
         gload(0);
         sstore(0);
@@ -186,15 +195,15 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
         floatx4 a[2][2], b[2][2];
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi) {
-            const float *p = &As[cur][(wm * 64 + mi * 32 + l32) * LDS32 + half * 8];
-            a[mi][0] = *(const floatx4 *)p;
-            a[mi][1] = *(const floatx4 *)(p + 4);
+            const float *p = &As[cur][(wm * 64 + mi * 32 + l32) * LDS32];
+            a[mi][0] = *(const floatx4 *)(p + c0);
+            a[mi][1] = *(const floatx4 *)(p + c1);
         }
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
-            const float *p = &Bs[cur][(wn * 64 + ni * 32 + l32) * LDS32 + half * 8];
-            b[ni][0] = *(const floatx4 *)p;
-            b[ni][1] = *(const floatx4 *)(p + 4);
+            const float *p = &Bs[cur][(wn * 64 + ni * 32 + l32) * LDS32];
+            b[ni][0] = *(const floatx4 *)(p + c0);
+            b[ni][1] = *(const floatx4 *)(p + c1);
         }
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
 
 // ---------------------------------------------------------------- split BF16
 constexpr int BK16 = 32;
-constexpr int LDS16 = BK16 + 8;  // row pitch (bf16): 80 bytes
+constexpr int LDS16 = BK16;  // row pitch (bf16): 64 bytes, chunk-swizzled like the FP32 tiles
 
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__ Ahi, const __bf16 *__restrict__ Alo,
@@ -236,8 +245,9 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
     const int nk = kend / BK16;
 
     // tile = 128 rows x 32 bf16 = 128 x 64 bytes: thread t moves 16 bytes at
-    // rows t/4 and t/4 + 64, columns 8 (t % 4) .. +7
+    // rows t/4 and t/4 + 64, columns 8 (t % 4) .. +7 (LDS chunk (t % 4) ^ swz)
     const int lr = tid >> 2, lc = (tid & 3) * 8;
+    const int ls = 8 * ((tid & 3) ^ swz(lr));
     const size_t a0 = (size_t)(m0 + lr) * lda + lc, a1 = (size_t)(m0 + lr + 64) * lda + lc;
     const size_t b0 = (size_t)(n0 + lr) * ldb + lc, b1 = (size_t)(n0 + lr + 64) * ldb + lc;
 
@@ -256,15 +266,15 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
         }
     };
     auto sstore = [&](int buf) {
-        *(bf16x8 *)(&Ash[buf][lr * LDS16 + lc]) = rah0;
-        *(bf16x8 *)(&Ash[buf][(lr + 64) * LDS16 + lc]) = rah1;
-        *(bf16x8 *)(&Bsh[buf][lr * LDS16 + lc]) = rbh0;
-        *(bf16x8 *)(&Bsh[buf][(lr + 64) * LDS16 + lc]) = rbh1;
+        *(bf16x8 *)(&Ash[buf][lr * LDS16 + ls]) = rah0;
+        *(bf16x8 *)(&Ash[buf][(lr + 64) * LDS16 + ls]) = rah1;
+        *(bf16x8 *)(&Bsh[buf][lr * LDS16 + ls]) = rbh0;
+        *(bf16x8 *)(&Bsh[buf][(lr + 64) * LDS16 + ls]) = rbh1;
         if (!plain) {
-            *(bf16x8 *)(&Asl[buf][lr * LDS16 + lc]) = ral0;
-            *(bf16x8 *)(&Asl[buf][(lr + 64) * LDS16 + lc]) = ral1;
-            *(bf16x8 *)(&Bsl[buf][lr * LDS16 + lc]) = rbl0;
-            *(bf16x8 *)(&Bsl[buf][(lr + 64) * LDS16 + lc]) = rbl1;
+            *(bf16x8 *)(&Asl[buf][lr * LDS16 + ls]) = ral0;
+            *(bf16x8 *)(&Asl[buf][(lr + 64) * LDS16 + ls]) = ral1;
+            *(bf16x8 *)(&Bsl[buf][lr * LDS16 + ls]) = rbl0;
+            *(bf16x8 *)(&Bsl[buf][(lr + 64) * LDS16 + ls]) = rbl1;
         }
     };
 
@@ -290,15 +300,16 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
             bf16x8 ah[2], bh[2], al[2], bl[2];
+            const int ck = 8 * ((2 * half + kk) ^ swz(l32));
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi) {
-                const int off = (wm * 64 + mi * 32 + l32) * LDS16 + half * 16 + kk * 8;
+                const int off = (wm * 64 + mi * 32 + l32) * LDS16 + ck;
                 ah[mi] = *(const bf16x8 *)(&Ash[cur][off]);
                 if (!plain) al[mi] = *(const bf16x8 *)(&Asl[cur][off]);
             }
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni) {
-                const int off = (wn * 64 + ni * 32 + l32) * LDS16 + half * 16 + kk * 8;
+                const int off = (wn * 64 + ni * 32 + l32) * LDS16 + ck;
                 bh[ni] = *(const bf16x8 *)(&Bsh[cur][off]);
                 if (!plain) bl[ni] = *(const bf16x8 *)(&Bsl[cur][off]);
             }
