@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate delayed-acceptance leg")
     ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -89,6 +91,22 @@ def copy_calibration(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+
+
+def valu_calibration():
+    """Measured VALU issue rates (lane-operations per cycle per SIMD at the
+    2.4 GHz peak clock) of the instruction kinds of the forward kernels."""
+    import ctypes
+    import torch
+    from altar_amd import _lib
+    torch.cuda.init()
+    names = ["fp32_fma", "fp32_fma_uniform", "fp32_mul", "fp32_add", "fp32_rcp", "fp64_fma"]
+    out = {}
+    for k, name in enumerate(names):
+        r = ctypes.c_double()
+        _lib.call("altar_valu_probe", k, ctypes.byref(r))
+        out[name] = {"lane_ops_per_s": r.value, "per_clk_per_simd": r.value / (1024 * 2.4e9)}
+    print(json.dumps({"valu_calibration": out}))
 
 
 def cpu_baseline(case, theta, seconds, omp=False):
@@ -122,6 +140,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return 0
+    if args.valu_calibration:
+        valu_calibration()
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
