@@ -16,7 +16,10 @@ and stores the whitened residuals of every exact evaluation (the training
 fields).  When beta reaches ``train_beta`` (or the store is full) the
 surrogate is fitted; afterwards each stage validates the surrogate on fresh
 exact fields of the current chains and falls back to the plain path when the
-field RMSE exceeds ``max_rmse`` metres.
+field RMSE exceeds ``max_rmse`` metres.  With ``retrain`` (the default) such
+a plain stage also replaces the stored fields by its own exact evaluations,
+which come from the current, narrower posterior, and the surrogate is
+refitted on them before the next stage's validation (retraining on drift).
 """
 import numpy as np
 import torch
@@ -28,9 +31,12 @@ from .metropolis import Metropolis
 
 class DelayedAcceptance(Metropolis):
     def __init__(self, surrogate, train_beta=0.1, max_rmse=2e-3, fit_kwargs=None, validate_chains=1024,
-                 **kwds):
+                 retrain=True, **kwds):
         super().__init__(**kwds)
         self.surrogate = surrogate
+        self.retrain = retrain
+        self._refit = False
+        self.refits = 0
         self.train_beta = train_beta
         self.max_rmse = max_rmse
         self.fit_kwargs = fit_kwargs or {}
@@ -49,12 +55,23 @@ class DelayedAcceptance(Metropolis):
         if not sur.trained and (step.beta >= self.train_beta or sur.space() < step.samples):
             if sur.count >= 1024:
                 sur.fit(**self.fit_kwargs)
+        elif self._refit and sur.count >= 1024:
+            # fields of the previous (plain) stage: refit on the current posterior
+            sur.fit(**self.fit_kwargs)
+            self._refit = False
+            self.refits += 1
         use_da = sur.trained and self._validate(annealer, step)
         self.mode_history.append({"beta": step.beta, "mode": "da" if use_da else "plain",
                                   "rmse": sur.validation_rmse})
         if use_da:
             return self._walk_da(annealer, step)
         if self._collecting(step):
+            return self._walk_plain(annealer, step, collect=True)
+        if sur.trained and self.retrain and step.samples % 128 == 0:
+            # the surrogate has drifted from the posterior: keep this stage's
+            # exact fields (replacing the stored ones) for a refit
+            sur.count = 0
+            self._refit = True
             return self._walk_plain(annealer, step, collect=True)
         stats = super().walkChains(annealer, step)
         self.exact_evals += step.samples * self.steps

@@ -700,8 +700,9 @@ int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C
 {
     return guarded([&] {
         require(wr_dev != nullptr && ldw >= h->Npad, "fields: output row pitch must be >= the padded N");
-        require(h->wkind == ALTAR_WHITEN_LOWER && !(h->model == ALTAR_LINEAR && h->prewhitened),
-                "fields: a lower-triangular whitener is required (linear sources in direct mode)");
+        require((h->wkind == ALTAR_WHITEN_LOWER || h->wkind == ALTAR_WHITEN_BLOCKS) &&
+                    !(h->model == ALTAR_LINEAR && h->prewhitened),
+                "fields: a lower-triangular or block whitener is required (linear sources in direct mode)");
         require(precision == ALTAR_FP32 || precision == ALTAR_BF16 || precision == ALTAR_BF16_REFINED,
                 "fields: precision must be FP32 or BF16");
         data_llk_impl(h, theta_dev, C, llk_dev, precision, wr_dev, ldw, S(stream));
@@ -785,9 +786,16 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         }
         int ntiles = ntn;
         if (blocked) {
-            require(!wr_out, "fields: a lower-triangular whitener is required");
             ntiles = 0;
-            for (const WBlock &B : h->blocks) ntiles += B.npad / 128;
+            int at = 0;
+            for (const WBlock &B : h->blocks) {
+                ntiles += B.npad / 128;
+                // fields: the blocks' padded column ranges must tile [0, Npad), so that
+                // every column of the caller's buffer is written
+                require(!wr_out || B.off == at, "fields: the whitening blocks must tile the padded observations");
+                at = B.off + B.npad;
+            }
+            require(!wr_out || at == (int)Np, "fields: the whitening blocks must tile the padded observations");
         }
         h->partial.reserve(Cp * ntiles * sizeof(float));
         GemmEpi ep{};
@@ -832,6 +840,7 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         auto whiten = [&](size_t off, int npad, const DevBuf &W32, const DevBuf &Whi, const DevBuf &Wlo, int tile0) {
             GemmEpi e = ep;
             e.partial = ep.partial + tile0;
+            if (wr_out) e.out = wr_out + off;  // this block's columns of the whitened fields
             if (!bf16)
                 hip_check(launch_gemm_f32(h->R32.as<float>() + off, Np, W32.as<float>(), (size_t)npad, (int)Mp, npad,
                                           npad, 1, epi, e, st),

@@ -241,6 +241,8 @@ class Surrogate:
 
     def field_rmse(self, wr_pred, wr_true):
         """RMSE of the LOS field in metres: W^-1 maps whitened residuals back."""
+        if getattr(self.model, "block_offsets", None) is not None:
+            return self._blocked_rmse(wr_pred - wr_true)
         L = getattr(self, "_L", None)
         if L is None and self.model.whitening_kind == "lower":
             # W = chol(Cd)^-1 for the exact whitener, so W^-1 = chol(Cd)
@@ -255,6 +257,31 @@ class Surrogate:
         else:
             f = diff @ L.T
         return float(np.sqrt(np.mean(f ** 2)))
+
+    def _blocked_rmse(self, diff):
+        """Block-diagonal whitener (one block per dataset): each dataset's
+        columns map back through its own W_b^-1 = chol(Cd_b) (exact mode)."""
+        from . import norms
+        blocks = getattr(self, "_Lb", None)
+        if blocks is None:
+            blocks = []
+            for off, ds in zip(self.model.block_offsets, self.model.datasets):
+                cd = np.asarray(ds.cd, dtype=np.float64)
+                if cd.ndim == 1:
+                    blocks.append((off, len(ds.x), np.sqrt(cd)))
+                else:
+                    L = ds.chol if ds.chol is not None else np.linalg.cholesky(cd)
+                    if getattr(self.model, "whiten", "exact") != "exact":
+                        L = np.linalg.inv(norms.whitener(cd, self.model.whiten)[1])
+                    blocks.append((off, len(ds.x), L))
+            self._Lb = blocks
+        sq, count = 0.0, 0
+        for off, n, L in blocks:
+            d = diff[:, off:off + n]
+            f = d * L[None, :] if L.ndim == 1 else d @ L.T
+            sq += float(np.sum(f ** 2))
+            count += f.size
+        return float(np.sqrt(sq / count))
 
     def validate_stored(self, idx):
         idx = np.sort(np.asarray(idx))[:4096]

@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate delayed-acceptance leg")
+    ap.add_argument("--joint-da", action="store_true",
+                    help="joint workload: also train the surrogate and time delayed acceptance against plain steps")
     ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
@@ -251,7 +253,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(case, theta, args.cpu_seconds)
         out["cpu_baseline_all_cores"] = cpu_baseline(case, theta, args.cpu_seconds / 2, omp=True)
     if not args.no_da:
-        out["surrogate_da"] = da_leg(args, case, model, acc / elapsed)
+        out["surrogate_da"] = da_leg(args, model, C)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -355,6 +357,12 @@ def side_workload(args):
         valid = max(C - invalid, 1)
         out["cdm_sides"] = {"fp64_threshold": model.fp64_sides, "fp32_per_chain": f32s / valid,
                             "fp64_per_chain": f64s / valid, "invalid_chains": invalid}
+    if args.workload == "joint" and args.joint_da:
+        # the surrogate covers the whole whitened residual field (both sources);
+        # 3 steps per stage, 196,608 training fields (3 exact steps of 65,536 chains)
+        out["surrogate_da"] = da_leg(args, model, C, seed=777 + 5, capacity=3 * C, stage_steps=3,
+                                     fit_kwargs=dict(epochs=20, batch=4096, lr=2e-3, pca_sample=16384),
+                                     validate_chains=512)
     if args.workload == "mogi":
         # the whole anneal from a fresh sample
         ann2 = Annealer(Metropolis(steps=5), COV(), seed=job.seed).initialize(job)
@@ -367,7 +375,7 @@ def side_workload(args):
     return out
 
 
-def da_leg(args, case, model, plain_acc_rate):
+def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024):
     """Anneal with field collection until beta >= 0.1, fit the surrogate
     (PCA of whitened residuals + BF16 MLP), then time delayed-acceptance steps
     and plain steps from the same state at the next temperature."""
@@ -377,18 +385,18 @@ def da_leg(args, case, model, plain_acc_rate):
     from altar_amd.bayesian.da import DelayedAcceptance
     from altar_amd.surrogate import Surrogate
 
-    C = args.chains
-    job = Job(chains=C, steps=5, seed=777 + 3)
+    job = Job(chains=C, steps=stage_steps, seed=seed)
     model.job = job
     t0 = time.perf_counter()
-    sur = Surrogate(model, K=64, capacity=args.da_train, max_batch=C, seed=3)
-    da = DelayedAcceptance(sur, train_beta=0.1, max_rmse=2e-3, steps=5,
-                           fit_kwargs=dict(epochs=40, batch=4096, lr=2e-3))
+    sur = Surrogate(model, K=64, capacity=capacity or args.da_train, max_batch=C, seed=3)
+    da = DelayedAcceptance(sur, train_beta=0.1, max_rmse=2e-3, steps=stage_steps, validate_chains=validate_chains,
+                           fit_kwargs=fit_kwargs or dict(epochs=40, batch=4096, lr=2e-3))
     ann = Annealer(da, COV(), seed=job.seed).initialize(job)
     ann.model = model
     w = ann.worker
     w.start(ann)
     t_fit = None
+    extra = 0
     while w.beta + job.tolerance < 1:
         w.cool(ann)
         stats = w.walk(ann)
@@ -396,7 +404,11 @@ def da_leg(args, case, model, plain_acc_rate):
         if sur.trained and t_fit is None:
             t_fit = time.perf_counter() - t0
         if sur.trained and w.beta >= 0.1:
-            break
+            # past training: stop once a stage ran delayed acceptance (the
+            # surrogate validated), or after a few refits on the posterior
+            extra += 1
+            if da.mode_history[-1]["mode"] == "da" or extra >= 4:
+                break
     setup = time.perf_counter() - t0
     if not sur.trained:
         return {"error": "surrogate not trained", "beta": w.beta}
@@ -428,6 +440,7 @@ def da_leg(args, case, model, plain_acc_rate):
     ann.sampler = da
     return {
         "beta": step.beta, "steps": K, "validation_rmse_m": sur.validation_rmse, "surrogate_used": bool(ok),
+        "refits": da.refits, "modes": [m["mode"] for m in da.mode_history],
         "training_fields": sur.count, "fit_s": t_fit, "setup_s": setup,
         "da": {"ms_per_step": t_da / K * 1e3, "accepted": acc_da, "exact_evals": exact,
                "accepted_samples_per_s": acc_da / t_da if t_da else None,
@@ -435,7 +448,7 @@ def da_leg(args, case, model, plain_acc_rate):
         "plain": {"ms_per_step": t_plain / K * 1e3, "accepted": acc_p, "exact_evals": C * K,
                   "accepted_samples_per_s": acc_p / t_plain},
         "gain_accepted_per_s": (acc_da / t_da) / (acc_p / t_plain) if (t_da and acc_p) else None,
-        "mlp_flops_per_chain": 2 * (10 * 256 + 256 * 256 * 2 + 256 * 64),
+        "mlp_flops_per_chain": 2 * (model.parameters * 256 + 256 * 256 * 2 + 256 * 64),
     }
 
 

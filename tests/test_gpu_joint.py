@@ -89,3 +89,65 @@ def test_joint_forward_and_llk(cuda, oracle):
     mask = torch.zeros(4, dtype=torch.int32, device=cuda)
     model.source.verify(torch.from_numpy(bad).to(cuda), mask)
     assert mask.cpu().numpy().tolist() == [1, 1, 1, 1]
+
+
+def test_joint_fields_and_surrogate(cuda, oracle):
+    """Whitened fields of the joint source with its block whitener (each
+    dataset's columns hold W_b r_b, padding columns zero) against the oracle,
+    and a surrogate fitted on them screening a short delayed-acceptance walk
+    (every move accounted for: accepted + rejected + unlikely = C x steps)."""
+    print("case: joint fields + surrogate DA, 2 x 900 InSAR + 40 GNSS, C=1024", flush=True)
+    from altar_amd import _lib
+    from altar_amd.bayesian import Annealer, COV
+    from altar_amd.bayesian.da import DelayedAcceptance
+    from altar_amd.surrogate import Surrogate
+    rng = np.random.default_rng(22)
+    fwd = oracle_forward(oracle)
+    case = cases.joint_case(n_insar=900, n_gnss=40, forward=fwd)
+    ds = case["datasets"]
+    C = 256
+    theta = joint_thetas(rng, oracle, case["truth"], C)
+    model = Joint(case["psets"], ds, job=Job(chains=C)).initialize()
+    t = torch.from_numpy(theta).to(cuda)
+    Np = (model.observations + 127) // 128 * 128
+    wr = torch.full((C, Np), 7.0, dtype=torch.float32, device=cuda)
+    llk = torch.empty(C, dtype=torch.float64, device=cuda)
+    _lib.call("altar_source_data_llk_fields", model.source.handle, _lib.ptr(t), C, _lib.ptr(llk), _lib.FP32,
+              _lib.ptr(wr), Np, None)
+    got = wr.cpu().numpy().astype(np.float64)
+    x = np.concatenate([d.x for d in ds]); y = np.concatenate([d.y for d in ds])
+    los = np.concatenate([d.los for d in ds])
+    oid = np.concatenate([np.full(len(d.x), k, np.int32) for k, d in enumerate(ds)])
+    res = fwd(theta, x, y, los, oid) - np.concatenate([d.d for d in ds])[None, :]
+    at = 0
+    real = np.zeros(Np, bool)
+    for off, d in zip(model.block_offsets, ds):
+        n = len(d.x)
+        cd = np.diag(d.cd) if np.ndim(d.cd) == 1 else d.cd
+        want = res[:, at:at + n] @ oracle.exact_whitener(cd).T
+        assert np.max(np.abs(got[:, off:off + n] - want)) < 1e-4 * np.max(np.abs(want))
+        real[off:off + n] = True
+        at += n
+    assert np.all(got[:, ~real] == 0.0)
+    # a short anneal with the surrogate screen
+    Cd = 1024
+    job = Job(chains=Cd, steps=3, seed=31)
+    model = Joint(case["psets"], ds, job=job).initialize()
+    sur = Surrogate(model, K=32, capacity=6144, max_batch=Cd, seed=1)
+    da = DelayedAcceptance(sur, train_beta=0.05, max_rmse=1.0, steps=3,
+                           fit_kwargs=dict(epochs=10, batch=1024, lr=2e-3, pca_sample=4096))
+    ann = Annealer(da, COV(), seed=job.seed).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    used = []
+    for _ in range(6):
+        w.cool(ann)
+        acc, rej, unl = w.walk(ann)
+        assert acc + rej + unl == Cd * da.steps
+        used.append(da.mode_history[-1]["mode"])
+        w.resample(ann, (acc, rej, unl))
+        if w.beta + job.tolerance >= 1:
+            break
+    print("joint DA modes", used, "validation rmse", sur.validation_rmse)
+    assert sur.trained and "da" in used
