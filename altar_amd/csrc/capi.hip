@@ -201,6 +201,11 @@ int altar_source_destroy(altar_source *h)
                          &h->Rlo, &h->R64, &h->partial, &h->T32, &h->Thi, &h->Tlo, &h->prm, &h->g32,
                          &h->g64, &h->flags, &h->shifts};
         for (DevBuf *b : all) b->release();
+        for (auto &slot : h->marks)
+            for (auto &p : slot) {
+                (void)hipEventDestroy(p.first);
+                (void)hipEventDestroy(p.second);
+            }
         h->census.release();
         h->fp64_list.release();
         h->fp64_count.release();
@@ -1182,12 +1187,14 @@ int altar_source_timers(altar_source *h, double *ms, long long *count, int nslot
 {
     return guarded([&] {
         require(h && ms && count && nslots > 0, "bad argument");
+        hipError_t first = hipSuccess;
         for (int s = 0; s < ALTAR_TIMER_SLOTS; ++s) {
             double tot = 0.0;
             for (auto &p : h->marks[s]) {
-                hip_check(hipEventSynchronize(p.second), "hipEventSynchronize");
                 float t = 0.0f;
-                hip_check(hipEventElapsedTime(&t, p.first, p.second), "hipEventElapsedTime");
+                hipError_t e = hipEventSynchronize(p.second);
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, p.first, p.second);
+                if (e != hipSuccess && first == hipSuccess) first = e;
                 tot += t;
                 (void)hipEventDestroy(p.first);
                 (void)hipEventDestroy(p.second);
@@ -1198,6 +1205,7 @@ int altar_source_timers(altar_source *h, double *ms, long long *count, int nslot
             }
             h->marks[s].clear();
         }
+        hip_check(first, "timing events");
     });
 }
 
