@@ -19,6 +19,8 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "bf16r": BF16_REFINED}
 
 # Philox stream tags (altar_amd/csrc/common.hpp)
 TAG_PROPOSAL, TAG_ACCEPT, TAG_RESAMPLE, TAG_INIT, TAG_SCREEN = 0, 1, 2, 3, 4
+TAG_TRAJ_PROPOSAL, TAG_TRAJ_ACCEPT = 5, 6
+TRAJ_MAX_SUBSTEPS = 1024   # sub-step streams use step' = step * 1024 + sub
 
 
 class AltarError(RuntimeError):
@@ -90,6 +92,12 @@ SIGNATURES = {
     "altar_da_pack": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "altar_da_accept": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _i, _u64, _u32, _u32,
                              _vp, _vp, _vp]),
+    "altar_da_trajectory_propose": (_i, [_vp, _i, _i, _vp, _u64, _u32, _u32, _u32, _vp, _vp, _vp]),
+    "altar_da_trajectory_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _i, _u64, _u32, _u32, _u32,
+                                      _vp, _vp, _vp]),
+    "altar_da_trajectory_list": (_i, [_vp, _i, _vp, _vp]),
+    "altar_da_trajectory_fused": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d,
+                                       _d, _i, _i, _u64, _u32, _u32, _i, _vp, _vp, _vp]),
 }
 
 

@@ -11,6 +11,19 @@ error; the exact forward model runs only for the chains that pass stage 1
 (compacted into a dense batch).  No reference counterpart: this extends
 altar/altar/bayesian/Metropolis.py:114-233.
 
+With ``substeps`` > 1 each exact step instead follows a surrogate
+trajectory (Liu 2001, "Monte Carlo Strategies in Scientific Computing",
+the surrogate transition method): ``substeps`` Metropolis steps of pi* lead
+from the state theta to an end point phi, which the exact model judges with
+the same ratio,
+
+    alpha = min(1, pi(phi) pi*(theta) / (pi(theta) pi*(phi))).
+
+A Metropolis kernel is reversible with respect to its own target, so its
+k-fold composition is too, and the ratio keeps pi invariant.  The exact
+model runs only for the chains whose trajectory moved; an accepted move has
+travelled up to ``substeps`` surrogate steps.
+
 Schedule: while beta < ``train_beta`` the sampler runs plain Metropolis steps
 and stores the whitened residuals of every exact evaluation (the training
 fields).  When beta reaches ``train_beta`` (or the store is full) the
@@ -31,9 +44,15 @@ from .metropolis import Metropolis
 
 class DelayedAcceptance(Metropolis):
     def __init__(self, surrogate, train_beta=0.1, max_rmse=2e-3, fit_kwargs=None, validate_chains=1024,
-                 retrain=True, **kwds):
+                 retrain=True, substeps=1, substep_scale=1.0, fused=True, **kwds):
         super().__init__(**kwds)
+        if not 1 <= substeps <= _lib.TRAJ_MAX_SUBSTEPS:
+            raise ValueError(f"substeps must lie in [1, {_lib.TRAJ_MAX_SUBSTEPS}]")
         self.surrogate = surrogate
+        self.substeps = substeps
+        self.substep_scale = substep_scale   # sub-step proposal = substep_scale * (L xi)
+        self.fused = fused                   # whole trajectories in one launch where supported
+        self.substep_counts = [0, 0, 0]   # surrogate sub-steps: accepted, rejected by the model, unlikely
         self.retrain = retrain
         self._refit = False
         self.refits = 0
@@ -64,6 +83,8 @@ class DelayedAcceptance(Metropolis):
         self.mode_history.append({"beta": step.beta, "mode": "da" if use_da else "plain",
                                   "rmse": sur.validation_rmse})
         if use_da:
+            if self.substeps > 1:
+                return self._walk_trajectory(annealer, step)
             return self._walk_da(annealer, step)
         if self._collecting(step):
             return self._walk_plain(annealer, step, collect=True)
@@ -117,6 +138,20 @@ class DelayedAcceptance(Metropolis):
         self.exact_evals += step.samples * self.steps
         return stats
 
+    def _workspace(self, C, P, dev):
+        f64 = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
+        ws = getattr(self, "_dws", None)
+        if ws is None or ws["cand"].shape != (C, P):
+            ws = {"cand": f64(C, P), "xi": f64(C, P), "cprior": f64(C), "cdata_s": f64(C), "cpost_s": f64(C),
+                  "post_s": f64(C), "llk_s": f64(C), "packed": f64(C, P), "cdata": f64(C),
+                  "phi": f64(C, P), "prior_phi": f64(C), "post_phi": f64(C),
+                  "mask": torch.empty(C, dtype=torch.int32, device=dev),
+                  "index": torch.empty(2 * C, dtype=torch.int32, device=dev),
+                  "count": torch.zeros(1, dtype=torch.int32, device=dev),
+                  "z": torch.empty((C, self.surrogate.K), dtype=torch.float32, device=dev)}
+            self._dws = ws
+        return ws
+
     def _walk_da(self, annealer, step):
         model = annealer.model
         sur = self.surrogate
@@ -124,16 +159,7 @@ class DelayedAcceptance(Metropolis):
         dev = step.theta.device
         seed = annealer.seed
         stage = annealer.worker.iteration
-        f64 = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
-        ws = getattr(self, "_dws", None)
-        if ws is None or ws["cand"].shape != (C, P):
-            ws = {"cand": f64(C, P), "xi": f64(C, P), "cprior": f64(C), "cdata_s": f64(C), "cpost_s": f64(C),
-                  "post_s": f64(C), "llk_s": f64(C), "packed": f64(C, P), "cdata": f64(C),
-                  "mask": torch.empty(C, dtype=torch.int32, device=dev),
-                  "index": torch.empty(2 * C, dtype=torch.int32, device=dev),
-                  "count": torch.zeros(1, dtype=torch.int32, device=dev),
-                  "z": torch.empty((C, sur.K), dtype=torch.float32, device=dev)}
-            self._dws = ws
+        ws = self._workspace(C, P, dev)
         s = _lib.stream()
         beta = float(step.beta)
         # surrogate posterior of the current states
@@ -169,3 +195,81 @@ class DelayedAcceptance(Metropolis):
         acc, rej, unl2, scr = (int(v) for v in counters.cpu().tolist())
         self.screened += scr
         return acc, rej, unl2 + scr
+
+    def fused_supported(self, model, P):
+        """Whether altar_da_trajectory_fused takes this model and network:
+        one model over the whole sample, at most 32 parameters, a network of
+        hidden width 256 with at most 64 inputs and outputs."""
+        m = self.surrogate.mlp
+        return bool(self.fused and P <= 32 and m.nhid == 256 and m.nin <= 64 and m.nout <= 64
+                    and 2 <= m.nlayers <= 8 and getattr(model, "offset", 0) == 0 and model.parameters == P
+                    and hasattr(model, "_prior") and hasattr(model, "source"))
+
+    def _walk_trajectory(self, annealer, step):
+        """Exact steps along surrogate trajectories of ``substeps`` sub-steps.
+
+        Returns (accepted, not moved or rejected, unlikely at the exact
+        stage) per exact step, so the proposal scaling follows the rate at
+        which chains change state.  With ``fused`` (and a model and network
+        the fused kernel supports) each trajectory is one launch
+        (altar_da_trajectory_fused); otherwise every sub-step is a sequence
+        of separate calls.  Both give the same trajectories."""
+        model = annealer.model
+        sur = self.surrogate
+        C, P = step.samples, step.parameters
+        dev = step.theta.device
+        seed = annealer.seed
+        stage = annealer.worker.iteration
+        ws = self._workspace(C, P, dev)
+        s = _lib.stream()
+        beta = float(step.beta)
+        p = _lib.ptr
+        fused = self.fused_supported(model, P)
+        if not fused:
+            sur.data_llk(step.theta, ws["llk_s"], ws["z"])
+            _lib.call("altar_posterior", p(step.prior), p(ws["llk_s"]), beta, C, p(ws["post_s"]), s)
+        counters = torch.zeros(4, dtype=torch.int64, device=dev)
+        sub_counters = torch.zeros(3, dtype=torch.int64, device=dev)
+        candidate = CoolingStep(beta, ws["cand"], (ws["cprior"], ws["cdata_s"], ws["cpost_s"]), sigma=step.sigma)
+        L = self.sigma_chol if self.substep_scale == 1.0 else (self.sigma_chol * self.substep_scale).contiguous()
+        d = annealer.dispatcher
+        moved_total = 0
+        for k in range(self.steps):
+            d.notify(d.chainAdvanceStart, annealer)
+            if fused:
+                _lib.call("altar_da_trajectory_fused", sur.mlp.handle, model._prior.handle, model.source.handle,
+                          p(step.theta), p(step.prior), p(ws["post_s"]), p(ws["phi"]), p(ws["prior_phi"]),
+                          p(ws["post_phi"]), p(L), p(sur.e_d), p(sur.s_d), sur.K, float(sur.cperp),
+                          float(model.normalization), beta, C, P, seed, stage, k, self.substeps, p(ws["index"]),
+                          p(sub_counters), s)
+            else:
+                ws["phi"].copy_(step.theta)
+                ws["prior_phi"].copy_(step.prior)
+                ws["post_phi"].copy_(ws["post_s"])
+            for j in range(0 if fused else self.substeps):
+                _lib.call("altar_da_trajectory_propose", p(ws["phi"]), C, P, p(L), seed, stage, k, j,
+                          p(ws["xi"]), p(ws["cand"]), s)
+                ws["mask"].zero_()
+                model.verify(candidate, ws["mask"])
+                _lib.call("altar_mh_restore", p(ws["cand"]), p(ws["phi"]), p(ws["mask"]), C, P, s)
+                model.priorLikelihood(candidate)
+                sur.data_llk(ws["cand"], ws["cdata_s"], ws["z"])
+                _lib.call("altar_da_trajectory_step", p(ws["phi"]), p(ws["prior_phi"]), p(ws["post_phi"]),
+                          p(ws["cand"]), p(ws["cprior"]), p(ws["cdata_s"]), p(ws["cpost_s"]), p(ws["mask"]), beta,
+                          C, P, seed, stage, k, j, p(ws["index"]), p(sub_counters), s)
+            _lib.call("altar_da_trajectory_list", p(ws["index"]), C, p(ws["count"]), s)
+            n = int(ws["count"].item())
+            if n > 0:
+                _lib.call("altar_da_pack", p(ws["phi"]), p(ws["index"]), n, P, p(ws["packed"]), s)
+                packed = ws["packed"][:n]
+                model.source.data_llk(model.restrict(packed), ws["cdata"][:n], model.precision)
+                _lib.call("altar_da_accept", p(step.theta), p(step.prior), p(step.data), p(step.posterior),
+                          p(ws["post_s"]), p(ws["phi"]), p(ws["prior_phi"]), p(ws["cdata"]), p(ws["post_phi"]),
+                          p(ws["index"]), n, beta, P, seed, stage, k, None, p(counters), s)
+            self.exact_evals += n
+            moved_total += n
+            d.notify(d.chainAdvanceFinish, annealer)
+        acc, _, unl2, _ = (int(v) for v in counters.cpu().tolist())
+        for i, v in enumerate(sub_counters.cpu().tolist()):
+            self.substep_counts[i] += int(v)
+        return acc, C * self.steps - moved_total, unl2

@@ -1510,6 +1510,95 @@ int altar_da_accept(double *theta_dev, double *prior_dev, double *data_dev, doub
     });
 }
 
+constexpr uint32_t TRAJ_MAX_SUBSTEPS = 1024;
+
+int altar_da_trajectory_propose(const double *phi_dev, int C, int P, const double *L_dev, uint64_t seed,
+                                uint32_t stage, uint32_t step, uint32_t sub, double *xi_dev, double *cand_dev,
+                                void *stream)
+{
+    return guarded([&] {
+        require(phi_dev && L_dev && xi_dev && cand_dev && C > 0 && P > 0, "bad argument");
+        require(sub < TRAJ_MAX_SUBSTEPS && step < (1u << 22), "sub-step index out of range");
+        hip_check(launch_da_traj_propose(phi_dev, C, P, L_dev, seed, stage, step * TRAJ_MAX_SUBSTEPS + sub, xi_dev,
+                                         cand_dev, S(stream)),
+                  "trajectory propose");
+    });
+}
+
+int altar_da_trajectory_step(double *phi_dev, double *prior_phi_dev, double *post_phi_dev, const double *cand_dev,
+                             const double *cprior_dev, const double *cdata_s_dev, double *cpost_s_dev,
+                             const int *mask_dev, double beta, int C, int P, uint64_t seed, uint32_t stage,
+                             uint32_t step, uint32_t sub, int *index_dev, unsigned long long *counters_dev,
+                             void *stream)
+{
+    return guarded([&] {
+        require(phi_dev && prior_phi_dev && post_phi_dev && cand_dev && cprior_dev && cdata_s_dev && cpost_s_dev &&
+                    mask_dev && index_dev && counters_dev && C > 0 && P > 0,
+                "bad argument");
+        require(sub < TRAJ_MAX_SUBSTEPS && step < (1u << 22), "sub-step index out of range");
+        hip_check(launch_da_traj_step(phi_dev, prior_phi_dev, post_phi_dev, cand_dev, cprior_dev, cdata_s_dev,
+                                      cpost_s_dev, mask_dev, beta, C, P, seed, stage,
+                                      step * TRAJ_MAX_SUBSTEPS + sub, sub == 0 ? 1 : 0, index_dev, counters_dev,
+                                      S(stream)),
+                  "trajectory step");
+    });
+}
+
+int altar_da_trajectory_fused(altar_mlp *mlp, const altar_prior *prior, const altar_source *src,
+                              const double *theta_dev, const double *prior_theta_dev, double *post_s_dev,
+                              double *phi_dev, double *prior_phi_dev, double *post_phi_dev, const double *L_dev,
+                              const double *e_dev, const double *s_dev, int K, double cperp, double normalization,
+                              double beta, int C, int P, uint64_t seed, uint32_t stage, uint32_t step,
+                              int substeps, int *index_dev, unsigned long long *counters_dev, void *stream)
+{
+    return guarded([&] {
+        require(mlp && prior && theta_dev && prior_theta_dev && post_s_dev && phi_dev && prior_phi_dev &&
+                    post_phi_dev && L_dev && e_dev && s_dev && index_dev && counters_dev && C > 0 && P > 0,
+                "bad argument");
+        require(P <= TRAJ_PMAX, "fused trajectories take at most 32 parameters");
+        require(P == prior->P && P == mlp->P, "parameter count differs from the prior's or the network's");
+        require(mlp->nhid == 256 && mlp->kin == 64 && mlp->kout == 64 && mlp->nlayers >= 2 &&
+                    mlp->nlayers <= TRAJ_MAX_LAYERS,
+                "fused trajectories need a network of hidden width 256, at most 64 inputs and outputs");
+        require(K > 0 && K <= mlp->nout, "K exceeds the network's outputs");
+        require(substeps >= 1 && (uint32_t)substeps <= TRAJ_MAX_SUBSTEPS && step < (1u << 22),
+                "sub-step count or step index out of range");
+        TrajArgs a{};
+        a.theta = theta_dev; a.prior_theta = prior_theta_dev; a.post_s = post_s_dev;
+        a.phi = phi_dev; a.prior_phi = prior_phi_dev; a.post_phi = post_phi_dev;
+        a.L = L_dev;
+        a.kind = prior->kind.as<int>(); a.pa = prior->a.as<double>(); a.pb = prior->b.as<double>();
+        a.has_cdm = 0;
+        if (src && (src->model == ALTAR_CDM || src->model == ALTAR_JOINT)) {
+            require(src->have_layout, "layout not set");
+            require(src->P == P, "the source's parameter count differs from the sample's");
+            a.has_cdm = 1;
+            a.cdm = cdm_layout(src);
+        }
+        for (int l = 0; l < mlp->nlayers; ++l) {
+            a.W[l] = mlp->W16[l].as<__bf16>();
+            a.bias[l] = mlp->b32[l].as<float>();
+        }
+        a.nlayers = mlp->nlayers;
+        a.lo = mlp->lo.as<double>(); a.hi = mlp->hi.as<double>();
+        a.e = e_dev; a.s = s_dev; a.cperp = cperp; a.norm = normalization; a.K = K;
+        a.beta = beta; a.C = C; a.P = P; a.seed = seed; a.stage = stage;
+        a.step2 = step * TRAJ_MAX_SUBSTEPS;
+        a.substeps = substeps;
+        a.moved = index_dev + C;
+        a.counters = counters_dev;
+        hip_check(launch_traj_fused(a, S(stream)), "fused trajectory");
+    });
+}
+
+int altar_da_trajectory_list(int *index_dev, int C, int *count_dev, void *stream)
+{
+    return guarded([&] {
+        require(index_dev && count_dev && C > 0, "bad argument");
+        hip_check(launch_da_traj_list(index_dev, C, count_dev, S(stream)), "trajectory list");
+    });
+}
+
 }  // extern "C"
 
 extern "C" {

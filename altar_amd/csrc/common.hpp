@@ -42,6 +42,8 @@ enum StreamTag : uint32_t {
     TAG_RESAMPLE = 2,   // COV multiplicity draws
     TAG_INIT = 3,       // initial sample from the prep distributions
     TAG_SCREEN = 4,     // delayed-acceptance first-stage dice
+    TAG_TRAJ_PROPOSAL = 5,  // surrogate-trajectory sub-step normals
+    TAG_TRAJ_ACCEPT = 6,    // surrogate-trajectory sub-step dice
 };
 
 __host__ __device__ inline u32x4 draw(uint64_t seed, uint32_t stage, uint32_t step,

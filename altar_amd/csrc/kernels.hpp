@@ -162,6 +162,43 @@ hipError_t launch_da_accept(double *theta, double *prior, double *data, double *
                             const double *cpost_s, const int *index, int n, double beta, int P, uint64_t seed,
                             uint32_t stage, uint32_t step, int *decision, unsigned long long *counters,
                             hipStream_t st);
+// surrogate trajectory (sampler.hip); step2 = step * 1024 + sub-step
+hipError_t launch_da_traj_propose(const double *phi, int C, int P, const double *L, uint64_t seed, uint32_t stage,
+                                  uint32_t step2, double *xi, double *cand, hipStream_t st);
+hipError_t launch_da_traj_step(double *phi, double *prior_phi, double *post_phi, const double *cand,
+                               const double *cprior, const double *cdata_s, double *cpost_s, const int *mask,
+                               double beta, int C, int P, uint64_t seed, uint32_t stage, uint32_t step2, int first,
+                               int *index, unsigned long long *counters, hipStream_t st);
+hipError_t launch_da_traj_list(int *index, int C, int *count, hipStream_t st);
+
+// the whole surrogate trajectory in one launch (trajectory.hip)
+constexpr int TRAJ_PMAX = 32;        // parameters per sample
+constexpr int TRAJ_MAX_LAYERS = 8;
+struct TrajArgs {
+    const double *theta, *prior_theta;   // C x P state and its prior llk
+    double *post_s;                      // out: surrogate posterior of the state
+    double *phi, *prior_phi, *post_phi;  // out: trajectory end points
+    const double *L;                     // P x P lower (proposal)
+    const int *kind;                     // prior: kind, a, b per parameter
+    const double *pa, *pb;
+    int has_cdm;                         // model check: CDM source above the free surface
+    CdmLayout cdm;
+    const __bf16 *W[TRAJ_MAX_LAYERS];    // layer l: dout x din (din 64 first, 256 after; dout 256, 64 last)
+    const float *bias[TRAJ_MAX_LAYERS];
+    int nlayers;
+    const double *lo, *hi;               // network input scaling
+    const double *e, *s;                 // surrogate llk: norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2
+    double cperp, norm;
+    int K;
+    double beta;
+    int C, P;
+    uint64_t seed;
+    uint32_t stage, step2;               // sub-step j draws at step2 + j
+    int substeps;
+    int *moved;                          // C: 1 if any sub-step moved the chain
+    unsigned long long *counters;        // [0] accepted, [1] rejected by the model, [2] unlikely
+};
+hipError_t launch_traj_fused(const TrajArgs &a, hipStream_t st);
 
 // cov.hip
 hipError_t launch_cov_update(const double *llk, int C, double beta, double tolerance, double target, double *w,

@@ -11,6 +11,7 @@
 // tile sizes by the host.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "chain_math.hpp"
 
 #include <hip/hip_bf16.h>
 
@@ -170,12 +171,7 @@ __global__ void surrogate_llk_kernel(const float *__restrict__ z, int ldz, int C
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    double acc = cperp;
-    for (int k = 0; k < K; ++k) {
-        const double t = e[k] + s[k] * (double)z[(size_t)c * ldz + k];
-        acc += t * t;
-    }
-    llk[c] = norm - 0.5 * acc;
+    llk[c] = surrogate_llk_sum(z + (size_t)c * ldz, e, s, K, cperp, norm);
 }
 
 // theta (C x P FP64) -> normalized network input (bf16, C x ldx, zero padded):

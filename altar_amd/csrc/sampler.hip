@@ -11,6 +11,7 @@
 // uniforms on (0, 1] with 53 bits.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "chain_math.hpp"
 
 // The state updates (posterior = prior + beta data, the accept test) follow
 // the reference's separate multiply and add; no fused multiply-add
@@ -45,11 +46,7 @@ __global__ void propose_kernel(const double *__restrict__ theta, const double *_
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)C * P) return;
     const int c = (int)(idx / P), i = (int)(idx % P);
-    const double *x = xi + (size_t)c * P;
-    const double *l = L + (size_t)i * P;
-    double acc = 0.0;
-    for (int j = 0; j <= i; ++j) acc += l[j] * x[j];
-    cand[idx] = theta[idx] + acc;
+    cand[idx] = propose_elem(theta[idx], L + (size_t)i * P, xi + (size_t)c * P, i);
 }
 
 // The same product for large P (the linear model), tiled through LDS: a
@@ -173,8 +170,7 @@ __global__ void prior_llk_kernel(const double *__restrict__ theta, int C, int P,
         if (kind[j] == 0) {
             acc += log(1.0 / (b[j] - a[j]));
         } else {
-            double z = (t[j] - a[j]) / b[j];
-            acc += -0.5 * z * z - log(b[j] * sqrt(2.0 * PI));
+            acc += gauss_log_term(t[j], a[j], b[j], log(b[j] * sqrt(2.0 * PI)));
         }
     }
     llk[c] = acc;
@@ -195,8 +191,7 @@ __global__ __launch_bounds__(256) void prior_llk_wave_kernel(const double *__res
         if (kind[j] == 0) {
             acc += log(1.0 / (b[j] - a[j]));
         } else {
-            double z = (t[j] - a[j]) / b[j];
-            acc += -0.5 * z * z - log(b[j] * sqrt(2.0 * PI));
+            acc += gauss_log_term(t[j], a[j], b[j], log(b[j] * sqrt(2.0 * PI)));
         }
     }
     acc = wave_sum(acc);
@@ -265,8 +260,7 @@ __global__ void accept_kernel(double *__restrict__ theta, double *__restrict__ p
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int dec = -1;
     if (c < C) {
-        const double bd = beta * cdata[c];
-        const double cp = cprior[c] + bd;
+        const double cp = tempered(cprior[c], beta, cdata[c]);
         if (cpost_out) cpost_out[c] = cp;
         if (mask[c]) {
             dec = 1;
@@ -320,8 +314,7 @@ __global__ void posterior_kernel(const double *__restrict__ prior, const double 
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < C) {
-        const double bd = beta * data[c];
-        post[c] = prior[c] + bd;
+        post[c] = tempered(prior[c], beta, data[c]);
     }
 }
 
@@ -550,8 +543,7 @@ __global__ void da_screen_kernel(const double *__restrict__ cprior, const double
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     int pass = 0, rej = 0, scr = 0;
     if (c < C) {
-        const double bd = beta * cdata_s[c];
-        const double cp = cprior[c] + bd;
+        const double cp = tempered(cprior[c], beta, cdata_s[c]);
         cpost_s[c] = cp;
         if (mask[c]) {
             rej = 1;
@@ -631,8 +623,7 @@ __global__ void da_accept_kernel(double *__restrict__ theta, double *__restrict_
     int acc = 0, unl = 0;
     if (j < n) {
         const int c = index[j];
-        const double bd = beta * cdata_packed[j];
-        const double cp = cprior[c] + bd;
+        const double cp = tempered(cprior[c], beta, cdata_packed[j]);
         u32x4 r = draw(seed, stage, step, (uint32_t)c, TAG_ACCEPT, 0u);
         const double u = u53(r.x, r.y);
         const double lhs = (cp - post[c]) - (cpost_s[c] - post_s[c]);
@@ -658,6 +649,81 @@ __global__ void da_accept_kernel(double *__restrict__ theta, double *__restrict_
         if (cnt[0]) atomicAdd(&counters[0], (unsigned long long)cnt[0]);
         if (cnt[1]) atomicAdd(&counters[2], (unsigned long long)cnt[1]);
     }
+}
+
+// Surrogate trajectory (Liu 2001, the surrogate transition method): each
+// exact step runs `substeps` Metropolis steps of the surrogate posterior
+// pi* = prior + beta llk_s from the chain's state, then judges the end point
+// phi with the exact posterior, accepting with
+//     min(1, pi(phi) pi*(theta) / (pi(theta) pi*(phi))).
+// A Metropolis kernel is reversible with respect to its own target, so a
+// k-fold composition of it is too, and this ratio keeps pi invariant.
+// One sub-step: cpost_s = cprior + beta cdata_s; accept phi <- cand if the
+// model accepts it and log u <= cpost_s - post_phi (u from the trajectory
+// dice stream at step' = step * 1024 + sub).  moved[c] records whether any
+// sub-step of this trajectory moved the chain (sub 0 resets it).
+// counters: [0] accepted, [1] rejected by the model, [2] unlikely.
+__global__ void da_traj_step_kernel(double *__restrict__ phi, double *__restrict__ prior_phi,
+                                    double *__restrict__ post_phi, const double *__restrict__ cand,
+                                    const double *__restrict__ cprior, const double *__restrict__ cdata_s,
+                                    double *__restrict__ cpost_s, const int *__restrict__ mask, double beta, int C,
+                                    int P, uint64_t seed, uint32_t stage, uint32_t step2, int first,
+                                    int *__restrict__ moved, unsigned long long *__restrict__ counters)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int acc = 0, rej = 0, unl = 0;
+    if (c < C) {
+        const double cp = tempered(cprior[c], beta, cdata_s[c]);
+        cpost_s[c] = cp;
+        if (mask[c]) {
+            rej = 1;
+        } else {
+            u32x4 r = draw(seed, stage, step2, (uint32_t)c, TAG_TRAJ_ACCEPT, 0u);
+            const double u = u53(r.x, r.y);
+            if (log(u) <= cp - post_phi[c]) {
+                acc = 1;
+                for (int k = 0; k < P; ++k) phi[(size_t)c * P + k] = cand[(size_t)c * P + k];
+                prior_phi[c] = cprior[c];
+                post_phi[c] = cp;
+            } else {
+                unl = 1;
+            }
+        }
+        moved[c] = first ? acc : (moved[c] | acc);
+    }
+    __shared__ unsigned cnt[3];
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    if (acc) atomicAdd(&cnt[0], 1u);
+    if (rej) atomicAdd(&cnt[1], 1u);
+    if (unl) atomicAdd(&cnt[2], 1u);
+    __syncthreads();
+    if (threadIdx.x < 3 && cnt[threadIdx.x]) atomicAdd(&counters[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+hipError_t launch_da_traj_propose(const double *phi, int C, int P, const double *L, uint64_t seed, uint32_t stage,
+                                  uint32_t step2, double *xi, double *cand, hipStream_t st)
+{
+    hipError_t e = launch_normals(xi, C, P, seed, stage, step2, TAG_TRAJ_PROPOSAL, st);
+    if (e != hipSuccess) return e;
+    return launch_propose(phi, L, xi, C, P, cand, st);
+}
+
+hipError_t launch_da_traj_step(double *phi, double *prior_phi, double *post_phi, const double *cand,
+                               const double *cprior, const double *cdata_s, double *cpost_s, const int *mask,
+                               double beta, int C, int P, uint64_t seed, uint32_t stage, uint32_t step2, int first,
+                               int *index, unsigned long long *counters, hipStream_t st)
+{
+    // the moved flags live in the upper half of `index` (2 C ints)
+    hipLaunchKernelGGL(da_traj_step_kernel, grid1(C), dim3(256), 0, st, phi, prior_phi, post_phi, cand, cprior,
+                       cdata_s, cpost_s, mask, beta, C, P, seed, stage, step2, first, index + C, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_da_traj_list(int *index, int C, int *count, hipStream_t st)
+{
+    hipLaunchKernelGGL(da_compact_kernel, dim3(1), dim3(1024), 0, st, index + C, C, index, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_da_screen(const double *cprior, const double *cdata_s, const double *post_s, const int *mask,

@@ -1,0 +1,305 @@
+// Surrogate trajectories in one launch (Liu 2001, the surrogate transition
+// method; no reference counterpart).
+//
+// A workgroup owns 64 chains and runs their whole trajectory: the surrogate
+// posterior of the starting state, then `substeps` Metropolis sub-steps of
+// pi* = prior + beta llk_s, each one
+//   normals -> proposal phi + L xi -> prior support and model check ->
+//   prior llk -> MLP (BF16 MFMA, FP32 accumulation) -> surrogate llk ->
+//   accept or keep phi,
+// with the chain state, the activations and the outputs in LDS.  Only the
+// weights come from memory (L2-resident; 300 KB for 10 -> 256 x 3 -> 64).
+//
+// Every step repeats the arithmetic of the separate kernels it replaces,
+// in the same order: normals_kernel, propose_kernel, prior_verify_kernel,
+// cdm_verify_kernel, restore_kernel, prior_llk_kernel, mlp_input_kernel,
+// mlp_gemm_kernel (the same MFMA sequence over k), surrogate_llk_kernel,
+// posterior_kernel and da_traj_step_kernel.  The two paths therefore give
+// the same trajectories bit for bit, and the surrogate posterior is one
+// function throughout, which the exact stage's ratio relies on.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "chain_math.hpp"
+
+#include <hip/hip_bf16.h>
+
+namespace altar {
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TT = 64;             // chains per workgroup
+constexpr int THREADS = 256;       // 4 wavefronts
+constexpr int HID = 256;           // hidden width
+constexpr int KIN = 64;            // padded input width
+constexpr int KOUT = 64;           // padded output width
+constexpr int APITCH = HID + 8;    // bf16 row pitch of the activation tile
+constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
+
+__device__ inline float gelu_t(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+// One dense layer on the 64-row tile: act (64 x DIN bf16) times W^T
+// (W: DOUT x DIN bf16, row-major) plus bias; GELU into act (hidden layers,
+// after every wave has read it) or FP32 into zt (the last layer).  For
+// DOUT = 256 a wave computes 64 rows x 64 columns (2 x 2 MFMA tiles), for
+// DOUT = 64 one 32 x 32 tile.  The k order is mlp_gemm_kernel's: chunks of
+// 32, within a chunk the lanes' k-sets half * 16 + kk * 8 for kk = 0, 1.
+template <int DIN, int DOUT, bool LAST>
+__device__ void tile_layer(__bf16 *act, float *zt, const __bf16 *__restrict__ W, const float *__restrict__ bias,
+                           int lane, int wid)
+{
+    constexpr int RT = DOUT == HID ? 2 : 1, CT = DOUT == HID ? 2 : 1;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int row0 = DOUT == HID ? 0 : 32 * (wid >> 1);
+    const int col0 = DOUT == HID ? 64 * wid : 32 * (wid & 1);
+    floatx16 acc[RT][CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[rt][ct][v] = 0.0f;
+    const __bf16 *wrow[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) wrow[ct] = W + (size_t)(col0 + ct * 32 + l32) * DIN + half * 16;
+    // weights one chunk ahead in registers
+    bf16x8 bn[2][CT];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) bn[kk][ct] = *(const bf16x8 *)(wrow[ct] + kk * 8);
+#pragma unroll
+    for (int k0 = 0; k0 < DIN; k0 += 32) {
+        bf16x8 b[2][CT];
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) b[kk][ct] = bn[kk][ct];
+        if (k0 + 32 < DIN) {
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) bn[kk][ct] = *(const bf16x8 *)(wrow[ct] + k0 + 32 + kk * 8);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            bf16x8 a[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+                a[rt] = *(const bf16x8 *)(act + (row0 + rt * 32 + l32) * APITCH + k0 + half * 16 + kk * 8);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[kk][ct], acc[rt][ct], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave has read act
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int col = col0 + ct * 32 + l32;
+            const float bc = bias[col];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = row0 + rt * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+                const float y = acc[rt][ct][v] + bc;
+                if constexpr (LAST)
+                    zt[row * (KOUT + 1) + col] = y;
+                else
+                    act[row * APITCH + col] = (__bf16)gelu_t(y);
+            }
+        }
+    __syncthreads();
+}
+
+struct TrajShared {
+    __bf16 act[TT * APITCH];
+    float zt[TT * (KOUT + 1)];
+    double phi[TT * PP], cand[TT * PP], xi[TT * PP];
+    double L[TRAJ_PMAX * TRAJ_PMAX];
+    double lo[TRAJ_PMAX], hi[TRAJ_PMAX], pa[TRAJ_PMAX], pb[TRAJ_PMAX], pconst[TRAJ_PMAX];
+    int kind[TRAJ_PMAX];
+    double e[KOUT], s[KOUT];
+    double prior_phi[TT], post_phi[TT], cprior[TT];
+    int valid[TT];
+};
+
+// network input of the tile's rows from a chain tile (rows past C are zero)
+__device__ void write_input(TrajShared &sh, const double *rows, int c0, const TrajArgs &a)
+{
+    for (int it = threadIdx.x; it < TT * KIN; it += THREADS) {
+        const int r = it / KIN, j = it % KIN;
+        float v = 0.0f;
+        if (j < a.P && c0 + r < a.C) v = (float)(2.0 * (rows[r * PP + j] - sh.lo[j]) / (sh.hi[j] - sh.lo[j]) - 1.0);
+        sh.act[r * APITCH + j] = (__bf16)v;
+    }
+}
+
+__device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
+{
+    __syncthreads();
+    tile_layer<KIN, HID, false>(sh.act, sh.zt, a.W[0], a.bias[0], lane, wid);
+    for (int l = 1; l < a.nlayers - 1; ++l) tile_layer<HID, HID, false>(sh.act, sh.zt, a.W[l], a.bias[l], lane, wid);
+    tile_layer<HID, KOUT, true>(sh.act, sh.zt, a.W[a.nlayers - 1], a.bias[a.nlayers - 1], lane, wid);
+}
+
+// surrogate_llk_kernel's sum for tile row r
+__device__ double surrogate_llk_row(const TrajShared &sh, int r, const TrajArgs &a)
+{
+    return surrogate_llk_sum(sh.zt + r * (KOUT + 1), sh.e, sh.s, a.K, a.cperp, a.norm);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
+{
+    __shared__ TrajShared sh;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int c0 = blockIdx.x * TT;
+    const int P = a.P;
+    for (int i = tid; i < P * P; i += THREADS) sh.L[i] = a.L[i];
+    if (tid < P) {
+        const int j = tid;
+        sh.lo[j] = a.lo[j];
+        sh.hi[j] = a.hi[j];
+        sh.kind[j] = a.kind[j];
+        sh.pa[j] = a.pa[j];
+        sh.pb[j] = a.pb[j];
+        // the data-independent term of prior_llk_kernel's density
+        sh.pconst[j] = a.kind[j] == 0 ? log(1.0 / (a.pb[j] - a.pa[j])) : log(a.pb[j] * sqrt(2.0 * PI));
+    }
+    for (int k = tid; k < a.K; k += THREADS) {
+        sh.e[k] = a.e[k];
+        sh.s[k] = a.s[k];
+    }
+    for (int it = tid; it < TT * P; it += THREADS) {
+        const int r = it / P, j = it % P;
+        if (c0 + r < a.C) sh.phi[r * PP + j] = a.theta[(size_t)(c0 + r) * P + j];
+    }
+    __syncthreads();
+    // the surrogate posterior of the starting state
+    write_input(sh, sh.phi, c0, a);
+    network(sh, a, lane, wid);
+    const int r = tid;  // the chain of a thread of wavefront 0
+    const int c = c0 + r;
+    const bool mine = tid < TT && c < a.C;
+    int moved = 0;
+    unsigned n_acc = 0, n_rej = 0, n_unl = 0;
+    if (mine) {
+        const double llk = surrogate_llk_row(sh, r, a);
+        const double post = tempered(a.prior_theta[c], a.beta, llk);
+        sh.prior_phi[r] = a.prior_theta[c];
+        sh.post_phi[r] = post;
+        a.post_s[c] = post;
+    }
+    const int pairs = (P + 1) / 2;
+    for (int sub = 0; sub < a.substeps; ++sub) {
+        const uint32_t step2 = a.step2 + (uint32_t)sub;
+        // normals (normals_kernel)
+        for (int it = tid; it < TT * pairs; it += THREADS) {
+            const int rr = it / pairs, w = it % pairs;
+            if (c0 + rr >= a.C) continue;
+            u32x4 rn = draw(a.seed, a.stage, step2, (uint32_t)(c0 + rr), TAG_TRAJ_PROPOSAL, (uint32_t)w);
+            double u1 = u53(rn.x, rn.y), u2 = u53(rn.z, rn.w);
+            double rad = sqrt(-2.0 * log(u1));
+            double sn, co;
+            sincos(2.0 * PI * u2, &sn, &co);
+            sh.xi[rr * PP + 2 * w] = rad * co;
+            if (2 * w + 1 < P) sh.xi[rr * PP + 2 * w + 1] = rad * sn;
+        }
+        __syncthreads();
+        // proposal (propose_kernel)
+        for (int it = tid; it < TT * P; it += THREADS) {
+            const int rr = it / P, i = it % P;
+            if (c0 + rr >= a.C) continue;
+            sh.cand[rr * PP + i] = propose_elem(sh.phi[rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
+        }
+        __syncthreads();
+        // support and model check, restore, prior (prior_verify_kernel,
+        // cdm_verify_kernel, restore_kernel, prior_llk_kernel)
+        if (mine) {
+            double *t = sh.cand + r * PP;
+            int bad = 0;
+            for (int j = 0; j < P; ++j)
+                if (sh.kind[j] == 0 && !(sh.pa[j] <= t[j] && t[j] <= sh.pb[j])) bad = 1;
+            if (a.has_cdm) {
+                const CdmLayout &cl = a.cdm;
+                double V[12][3];
+                if (cdm::make_vertices(t[cl.xIdx], t[cl.xIdx + 1], t[cl.dIdx], t[cl.aXIdx], t[cl.aXIdx + 1],
+                                       t[cl.aXIdx + 2], t[cl.omegaXIdx], t[cl.omegaXIdx + 1], t[cl.omegaXIdx + 2],
+                                       V))
+                    bad = 1;
+            }
+            sh.valid[r] = !bad;
+            if (bad)
+                for (int j = 0; j < P; ++j) t[j] = sh.phi[r * PP + j];
+            double acc = 0.0;
+            for (int j = 0; j < P; ++j) {
+                if (sh.kind[j] == 0) {
+                    acc += sh.pconst[j];
+                } else {
+                    acc += gauss_log_term(t[j], sh.pa[j], sh.pb[j], sh.pconst[j]);
+                }
+            }
+            sh.cprior[r] = acc;
+        }
+        __syncthreads();
+        write_input(sh, sh.cand, c0, a);
+        network(sh, a, lane, wid);
+        // sub-step acceptance (da_traj_step_kernel)
+        if (mine) {
+            const double cdata_s = surrogate_llk_row(sh, r, a);
+            const double cp = tempered(sh.cprior[r], a.beta, cdata_s);
+            if (!sh.valid[r]) {
+                ++n_rej;
+            } else {
+                u32x4 rn = draw(a.seed, a.stage, step2, (uint32_t)c, TAG_TRAJ_ACCEPT, 0u);
+                const double u = u53(rn.x, rn.y);
+                if (log(u) <= cp - sh.post_phi[r]) {
+                    ++n_acc;
+                    moved = 1;
+                    for (int j = 0; j < P; ++j) sh.phi[r * PP + j] = sh.cand[r * PP + j];
+                    sh.prior_phi[r] = sh.cprior[r];
+                    sh.post_phi[r] = cp;
+                } else {
+                    ++n_unl;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int it = tid; it < TT * P; it += THREADS) {
+        const int rr = it / P, j = it % P;
+        if (c0 + rr < a.C) a.phi[(size_t)(c0 + rr) * P + j] = sh.phi[rr * PP + j];
+    }
+    if (mine) {
+        a.prior_phi[c] = sh.prior_phi[r];
+        a.post_phi[c] = sh.post_phi[r];
+        a.moved[c] = moved;
+    }
+    if (wid == 0) {
+        unsigned long long s0 = n_acc, s1 = n_rej, s2 = n_unl;
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off, 64);
+            s1 += __shfl_xor(s1, off, 64);
+            s2 += __shfl_xor(s2, off, 64);
+        }
+        if (lane == 0) {
+            if (s0) atomicAdd(&a.counters[0], s0);
+            if (s1) atomicAdd(&a.counters[1], s1);
+            if (s2) atomicAdd(&a.counters[2], s2);
+        }
+    }
+}
+
+hipError_t launch_traj_fused(const TrajArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(traj_fused_kernel, dim3((a.C + TT - 1) / TT), dim3(THREADS), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace altar

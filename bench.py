@@ -75,6 +75,11 @@ def parse():
     ap.add_argument("--joint-da", action="store_true",
                     help="joint workload: also train the surrogate and time delayed acceptance against plain steps")
     ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
+    ap.add_argument("--da-substeps", default="16",
+                    help="surrogate sub-steps per exact step in the trajectory leg, or a comma-separated "
+                         "list (the first is the reported one; 1 disables the leg)")
+    ap.add_argument("--da-substep-scale", type=float, default=1.0,
+                    help="sub-step proposal scale relative to the Metropolis proposal")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -375,10 +380,32 @@ def side_workload(args):
     return out
 
 
+class JumpMonitor:
+    """Squared jump distance of every chain per step, in units of the
+    starting sample's standard deviation per parameter, summed over the
+    parameters and averaged over the chains (expected squared jump distance,
+    Pasarica and Gelman 2010)."""
+
+    def __init__(self, step, sd):
+        self.step, self.inv_sd, self.prev, self.total, self.steps = step, 1.0 / sd, None, 0.0, 0
+
+    def on(self, event, controller):
+        if event == "chainAdvanceStart":
+            self.prev = self.step.theta.clone()
+        elif event == "chainAdvanceFinish" and self.prev is not None:
+            dx = (self.step.theta - self.prev) * self.inv_sd
+            self.total += float((dx * dx).sum(1).mean())
+            self.steps += 1
+
+
 def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024):
     """Anneal with field collection until beta >= 0.1, fit the surrogate
-    (PCA of whitened residuals + BF16 MLP), then time delayed-acceptance steps
-    and plain steps from the same state at the next temperature."""
+    (PCA of whitened residuals + BF16 MLP), then time three samplers from the
+    same state at the next temperature: plain Metropolis, two-stage delayed
+    acceptance, and delayed acceptance along surrogate trajectories of
+    --da-substeps sub-steps.  Each is timed on its own; a second, identical
+    run of each (same streams, same result) measures its squared jump
+    distance per step."""
     import torch
     from altar_amd import Job
     from altar_amd.bayesian import Annealer, COV, Metropolis
@@ -412,45 +439,96 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     setup = time.perf_counter() - t0
     if not sur.trained:
         return {"error": "surrogate not trained", "beta": w.beta}
-    # one more stage: temperature, covariance, resampling; then time both samplers
+    # one more stage: temperature, covariance, resampling; then time the samplers
     w.cool(ann)
     step = w.step
     da.prepareSamplingPDF(ann, step)
     ok = da._validate(ann, step)
-    state = step.clone()
+    start = step.clone()
+    sd = start.theta.std(0)
     K = max(2, args.steps)
     da.steps = K
-    torch.cuda.synchronize()
-    ev0 = da.exact_evals
-    t1 = time.perf_counter()
-    acc_da, rej_da, unl_da = da._walk_da(ann, step) if ok else (0, 0, 0)
-    torch.cuda.synchronize()
-    t_da = time.perf_counter() - t1
-    exact = da.exact_evals - ev0
-    # plain Metropolis from the same state and temperature
     plain = Metropolis(steps=K)
     plain.scaling = da.scaling
     plain.sigma_chol = da.sigma_chol
-    ann.sampler = plain
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    acc_p, rej_p, unl_p = plain.walkChains(ann, state)
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t2
+    sweep = [max(1, int(v)) for v in str(args.da_substeps).split(",")]
+    subs = sweep[0]
+    da.substep_scale = args.da_substep_scale
+
+    def walk(kind):
+        if kind == "plain":
+            ann.sampler = plain
+            return lambda st: plain.walkChains(ann, st)
+        ann.sampler = da
+        da.substeps = subs if kind == "trajectory" else 1
+        return (lambda st: da._walk_trajectory(ann, st)) if kind == "trajectory" else (lambda st: da._walk_da(ann, st))
+
+    def run(kind):
+        st = start.clone()
+        fn = walk(kind)
+        ev0 = da.exact_evals
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        acc, rej, unl = fn(st)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        exact = C * K if kind == "plain" else da.exact_evals - ev0
+        # replay with the jump monitor (same streams: the same moves)
+        st2 = start.clone()
+        mon = JumpMonitor(st2, sd)
+        ann.dispatcher.subscribe(mon)
+        try:
+            walk(kind)(st2)
+        finally:
+            ann.dispatcher.monitors.remove(mon)
+        same = bool(torch.equal(st2.theta, st.theta))
+        return {"ms_per_step": dt / K * 1e3, "accepted": acc, "exact_evals": exact,
+                "accepted_samples_per_s": acc / dt, "exact_evals_per_s": exact / dt,
+                "esjd_per_step": mon.total / max(mon.steps, 1), "esjd_per_s": mon.total / dt,
+                "replay_identical": same, "final_means": st.theta.mean(0).tolist(),
+                "final_sds": st.theta.std(0).tolist()}
+
+    res = {"plain": run("plain")}
+    if ok:
+        res["da"] = run("da")
+        runs = []
+        for n in sweep:
+            if n < 2:
+                continue
+            subs = n
+            sub0 = list(da.substep_counts)
+            r = run("trajectory")
+            sc = [b - a for a, b in zip(sub0, da.substep_counts)]
+            r.update(substeps=n, substep_scale=da.substep_scale, substep_acceptance=sc[0] / max(sum(sc), 1))
+            runs.append(r)
+        if runs:
+            res["trajectory"] = runs[0]
+        if len(runs) > 1:
+            res["trajectory_sweep"] = runs
     ann.sampler = da
-    return {
+    pl = res["plain"]
+    out = {
         "beta": step.beta, "steps": K, "validation_rmse_m": sur.validation_rmse, "surrogate_used": bool(ok),
         "refits": da.refits, "modes": [m["mode"] for m in da.mode_history],
         "training_fields": sur.count, "fit_s": t_fit, "setup_s": setup,
-        "da": {"ms_per_step": t_da / K * 1e3, "accepted": acc_da, "exact_evals": exact,
-               "accepted_samples_per_s": acc_da / t_da if t_da else None,
-               "exact_evals_per_s": exact / t_da if t_da else None},
-        "plain": {"ms_per_step": t_plain / K * 1e3, "accepted": acc_p, "exact_evals": C * K,
-                  "accepted_samples_per_s": acc_p / t_plain},
-        "gain_accepted_per_s": (acc_da / t_da) / (acc_p / t_plain) if (t_da and acc_p) else None,
         "mlp_flops_per_chain": 2 * (model.parameters * 256 + 256 * 256 * 2 + 256 * 64),
     }
-
+    out.update(res)
+    for kind, r in [("da", res.get("da")), ("trajectory", res.get("trajectory"))] + \
+            [(f"sweep{i}", r) for i, r in enumerate(res.get("trajectory_sweep", []))]:
+        if r is not None and pl["accepted"]:
+            r["gain_accepted_per_s"] = r["accepted_samples_per_s"] / pl["accepted_samples_per_s"]
+            r["gain_esjd_per_s"] = r["esjd_per_s"] / pl["esjd_per_s"] if pl["esjd_per_s"] else None
+            # final sample means against plain Metropolis's, in standard errors
+            ma, mp = np.array(r["final_means"]), np.array(pl["final_means"])
+            se = np.sqrt((np.array(r["final_sds"]) ** 2 + np.array(pl["final_sds"]) ** 2) / C)
+            r["max_mean_difference_se"] = float(np.max(np.abs(ma - mp) / se))
+    for kind in ("da", "trajectory"):
+        if kind in res and "gain_accepted_per_s" in res[kind]:
+            out[f"{kind}_gain_accepted_per_s"] = res[kind]["gain_accepted_per_s"]
+            out[f"{kind}_gain_esjd_per_s"] = res[kind]["gain_esjd_per_s"]
+    out["gain_accepted_per_s"] = out.get("da_gain_accepted_per_s")
+    return out
 
 if __name__ == "__main__":
     sys.exit(main())

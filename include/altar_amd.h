@@ -290,6 +290,49 @@ int altar_da_accept(double *theta_dev, double *prior_dev, double *data_dev, doub
                     uint32_t stage, uint32_t step, int *decision_dev, unsigned long long *counters_dev,
                     void *stream);
 
+/* ---- surrogate trajectories (Liu 2001, the surrogate transition method; no
+ * reference counterpart).  Each exact step runs `substeps` Metropolis steps of
+ * the surrogate posterior pi* = prior + beta llk_s from the chain's state theta
+ * to an end point phi, then altar_da_accept judges phi with the exact model:
+ * accept if log u2 <= (post(phi) - post(theta)) - (post_s(phi) - post_s(theta)).
+ * The ratio keeps the exact posterior invariant (a Metropolis kernel is
+ * reversible with respect to its target, so are its powers).
+ * Sub-step streams: step' = step * 1024 + sub (sub < 1024, step < 2^22). */
+/* cand = phi + (L xi)^T with xi from the trajectory proposal stream */
+int altar_da_trajectory_propose(const double *phi_dev, int C, int P, const double *L_dev, uint64_t seed,
+                                uint32_t stage, uint32_t step, uint32_t sub, double *xi_dev, double *cand_dev,
+                                void *stream);
+/* one surrogate Metropolis sub-step on (phi, prior_phi, post_phi):
+ * cpost_s = cprior + beta cdata_s; accept cand if mask[c] == 0 and
+ * log u <= cpost_s - post_phi (u: trajectory dice stream).  The upper half of
+ * index_dev (2 C ints) flags the chains that moved in this trajectory (sub 0
+ * resets the flags).  counters_dev[0] += accepted, [1] += rejected by the
+ * model, [2] += unlikely. */
+int altar_da_trajectory_step(double *phi_dev, double *prior_phi_dev, double *post_phi_dev, const double *cand_dev,
+                             const double *cprior_dev, const double *cdata_s_dev, double *cpost_s_dev,
+                             const int *mask_dev, double beta, int C, int P, uint64_t seed, uint32_t stage,
+                             uint32_t step, uint32_t sub, int *index_dev, unsigned long long *counters_dev,
+                             void *stream);
+/* The whole trajectory of every chain in one launch (a workgroup runs 64
+ * chains through all sub-steps with their state and the network's
+ * activations in LDS).  Sub-step for sub-step the same arithmetic as the
+ * separate calls: altar_da_trajectory_propose, the prior's support check,
+ * the source's check (CDM and joint sources; src may be NULL), the prior
+ * llk, the network (altar_mlp_forward), altar_surrogate_llk and
+ * altar_da_trajectory_step, so the end points agree bit for bit.
+ * Also writes post_s_dev[c] = prior_theta[c] + beta llk_s(theta_c), the
+ * surrogate posterior of the starting state for the exact stage.
+ * Needs P <= 32 and a network of hidden width 256 with at most 64 inputs and
+ * outputs.  Moved flags and counters as altar_da_trajectory_step. */
+int altar_da_trajectory_fused(altar_mlp *mlp, const altar_prior *prior, const altar_source *src,
+                              const double *theta_dev, const double *prior_theta_dev, double *post_s_dev,
+                              double *phi_dev, double *prior_phi_dev, double *post_phi_dev, const double *L_dev,
+                              const double *e_dev, const double *s_dev, int K, double cperp, double normalization,
+                              double beta, int C, int P, uint64_t seed, uint32_t stage, uint32_t step,
+                              int substeps, int *index_dev, unsigned long long *counters_dev, void *stream);
+/* index_dev[0..count) = the chains whose trajectory moved, in chain order */
+int altar_da_trajectory_list(int *index_dev, int C, int *count_dev, void *stream);
+
 /* Philox normals and uniforms into device memory (the same streams the
  * kernels draw), for parity checks */
 /* VALU throughput calibration (no reference counterpart): the sustained

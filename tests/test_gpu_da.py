@@ -7,6 +7,8 @@
    target's mean (within 5 Monte-Carlo standard errors) and covariance
    (within 5%): the two-stage rule keeps the target invariant whatever the
    surrogate.
+   The same holds for surrogate trajectories (several surrogate Metropolis
+   sub-steps per exact step, judged at the end point).
 2. Linear-Gaussian, annealed end to end with a surrogate trained on the
    annealer's own beta < train_beta evaluations: the DA posterior agrees with
    the closed form Sigma = (G^T Cd^-1 G + I)^-1, mu = Sigma G^T Cd^-1 d
@@ -75,6 +77,57 @@ def test_da_kernels_keep_target_invariant(cuda):
     assert np.allclose(np.cov(x.T), S, rtol=0.05, atol=0.05 * S.max())
 
 
+def test_trajectory_kernels_keep_target_invariant(cuda):
+    print("case: surrogate trajectories, 2-D Gaussian, C=8192, 60 steps x 4 sub-steps", flush=True)
+    rng = np.random.default_rng(32)
+    C, P, steps, subs = 8192, 2, 60, 4
+    mu = np.array([1.0, -2.0])
+    S = np.array([[2.0, 0.8], [0.8, 1.0]])
+    mu_s, S_s = mu + np.array([0.7, -0.5]), 1.5 * S        # biased surrogate
+    theta = rng.multivariate_normal(mu, S, C)
+    f = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(cuda)
+    th = f(theta)
+    prior = f(np.zeros(C))
+    data = f(gauss_logpdf(theta, mu, S))
+    post = f(data.cpu().numpy())
+    post_s = f(gauss_logpdf(theta, mu_s, S_s))
+    L = f(0.9 * np.linalg.cholesky(S))
+    xi, cand = torch.empty_like(th), torch.empty_like(th)
+    mask = torch.zeros(C, dtype=torch.int32, device=cuda)
+    cprior = f(np.zeros(C))
+    cpost_s = torch.empty(C, dtype=torch.float64, device=cuda)
+    phi, prior_phi, post_phi = torch.empty_like(th), torch.empty_like(prior), torch.empty_like(post)
+    index = torch.empty(2 * C, dtype=torch.int32, device=cuda)
+    count = torch.zeros(1, dtype=torch.int32, device=cuda)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    sub_counters = torch.zeros(3, dtype=torch.int64, device=cuda)
+    p = _lib.ptr
+    for k in range(steps):
+        phi.copy_(th)
+        prior_phi.copy_(prior)
+        post_phi.copy_(post_s)
+        for j in range(subs):
+            _lib.call("altar_da_trajectory_propose", p(phi), C, P, p(L), 17, 1, k, j, p(xi), p(cand), None)
+            cdata_s = f(gauss_logpdf(cand.cpu().numpy(), mu_s, S_s))
+            _lib.call("altar_da_trajectory_step", p(phi), p(prior_phi), p(post_phi), p(cand), p(cprior), p(cdata_s),
+                      p(cpost_s), p(mask), 1.0, C, P, 17, 1, k, j, p(index), p(sub_counters), None)
+        _lib.call("altar_da_trajectory_list", p(index), C, p(count), None)
+        n = int(count.item())
+        idx = index[:n].cpu().numpy()
+        cdata = f(gauss_logpdf(phi.cpu().numpy()[idx], mu, S))
+        _lib.call("altar_da_accept", p(th), p(prior), p(data), p(post), p(post_s), p(phi), p(prior_phi), p(cdata),
+                  p(post_phi), p(index), n, 1.0, P, 17, 1, k, None, p(counters), None)
+    x = th.cpu().numpy()
+    acc, _, unl2, _ = counters.cpu().numpy()
+    sacc, srej, sunl = sub_counters.cpu().numpy()
+    print("accepted", acc, "exact rejections", unl2, "sub-steps accepted", sacc, "of", sacc + srej + sunl,
+          "mean", x.mean(0), "cov", np.cov(x.T))
+    assert acc > 0.2 * C * steps and unl2 > 0 and srej == 0 and sacc + sunl == C * steps * subs
+    se = np.sqrt(np.diag(S) / C)
+    assert np.all(np.abs(x.mean(0) - mu) < 5 * se)
+    assert np.allclose(np.cov(x.T), S, rtol=0.05, atol=0.05 * S.max())
+
+
 def linear_problem(seed=12, N=64, P=3):
     rng = np.random.default_rng(seed)
     G = rng.normal(size=(N, P))
@@ -119,3 +172,76 @@ def test_da_linear_gaussian_posterior(cuda):
     assert np.all(np.abs(th.mean(0) - mu) < 5 * se)
     assert np.all(np.abs(th.std(0) / sd - 1) < 0.1)
     assert da.exact_evals / max(da_acc, 1) < plain_evals / max(plain_acc, 1)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_trajectory_linear_gaussian_posterior(cuda, fused):
+    print(f"case: linear-Gaussian surrogate trajectories C=8192 N=64 P=3 steps=10 sub-steps=6 fused={fused}",
+          flush=True)
+    G, d, cd, S, mu = linear_problem()
+    P, C, steps = 3, 8192, 10
+    job = Job(chains=C, steps=steps, seed=4, tolerance=1e-3)
+    model = Linear(default_psets(P), G, d, cd, mode="direct", job=job).initialize()
+    sur = Surrogate(model, K=32, capacity=C * steps * 2, max_batch=C, seed=1)
+    da = DelayedAcceptance(sur, train_beta=0.05, max_rmse=1e9, steps=steps, substeps=6, fused=fused,
+                           fit_kwargs=dict(epochs=30, batch=4096, lr=2e-3))
+    assert da.fused_supported(model, P) == fused
+    ann = Annealer(da, COV(), seed=4).initialize(job)
+    ann.posterior(model)
+    th = ann.worker.step.theta.cpu().numpy()
+    modes = [m["mode"] for m in da.mode_history]
+    print("surrogate field rmse", sur.validation_rmse, "modes", modes, "exact evals", da.exact_evals,
+          "sub-steps (accepted, rejected, unlikely)", da.substep_counts)
+    sd = np.sqrt(np.diag(S))
+    se = sd / np.sqrt(C / 10)
+    print("trajectory DA mean", th.mean(0), "closed form", mu, "se", se, "sd ratio", th.std(0) / sd)
+    assert sur.trained and "da" in modes and da.substep_counts[0] > 0
+    assert np.all(np.abs(th.mean(0) - mu) < 5 * se)
+    assert np.all(np.abs(th.std(0) / sd - 1) < 0.1)
+
+
+def test_fused_trajectory_matches_separate_calls(cuda):
+    """One launch per trajectory (altar_da_trajectory_fused) against the
+    separate calls, CDM source (free-surface check), two exact steps of 7
+    sub-steps with wide sub-step proposals: states, likelihoods, end points
+    and counters agree bit for bit."""
+    print("case: fused vs separate trajectories, CDM 1000 chains x 512 LOS, 2 steps x 7 sub-steps", flush=True)
+    from altar_amd.cases import cdm_case
+    from altar_amd.cdm import CDM
+    case = cdm_case(N=512)
+    C, P = 1000, 10
+    job = Job(chains=C, steps=1, seed=5)
+    model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"], nu=0.25,
+                job=job).initialize()
+    ann = Annealer(Metropolis(), COV(), seed=5).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    w.cool(ann)
+    step = w.step
+    ann.sampler.prepareSamplingPDF(ann, step)
+    sur = Surrogate(model, K=64, capacity=1024, max_batch=C, seed=2)
+    dev = step.theta.device
+    rng = np.random.default_rng(3)
+    sur.e_d = torch.from_numpy(rng.normal(0.0, 1.0, 64)).to(dev)
+    sur.s_d = torch.from_numpy(rng.uniform(0.05, 0.2, 64)).to(dev)
+    sur.cperp = 3.0
+    sur.trained = True
+    runs = []
+    for fused in (False, True):
+        da = DelayedAcceptance(sur, steps=2, substeps=7, substep_scale=3.0, fused=fused)
+        assert da.fused_supported(model, P) == fused
+        da.sigma_chol = ann.sampler.sigma_chol
+        st = step.clone()
+        stats = da._walk_trajectory(ann, st)
+        torch.cuda.synchronize()
+        runs.append((stats, st, list(da.substep_counts), {k: v.clone() for k, v in da._dws.items()}))
+    (s0, st0, c0, w0), (s1, st1, c1, w1) = runs
+    print("exact-stage stats", s0, s1, "sub-steps (accepted, rejected, unlikely)", c0, c1)
+    assert s0 == s1 and c0 == c1
+    assert c0[0] > 0 and c0[1] > 0 and c0[2] > 0 and s0[0] > 0
+    for name in ("theta", "prior", "data", "posterior"):
+        assert torch.equal(getattr(st0, name), getattr(st1, name)), name
+    for name in ("phi", "prior_phi", "post_phi", "post_s"):
+        assert torch.equal(w0[name], w1[name]), name
+    assert torch.equal(w0["index"][C:], w1["index"][C:])

@@ -136,3 +136,63 @@ def test_delayed_acceptance_rule(cuda, oracle):
     cnt = counters.cpu().numpy()
     assert cnt[0] == acc.sum() and cnt[1] == mask.sum()
     assert cnt[3] == ((mask == 0) & ~passed).sum() and cnt[2] == len(idx) - acc.sum()
+
+
+def test_trajectory_kernels_match_restatement(cuda, oracle):
+    """Sub-step proposal, surrogate Metropolis sub-step, moved list: bit-exact
+    against a numpy restatement on the oracle's Philox streams."""
+    print("case: trajectory kernels C=3001 P=7", flush=True)
+    rng = np.random.default_rng(44)
+    C, P, beta = 3001, 7, 0.37
+    seed, stage, step = 21, 5, 3
+    phi0 = rng.normal(size=(C, P))
+    A = rng.normal(size=(P, P)) * 0.1
+    L = np.linalg.cholesky(A @ A.T + 0.05 * np.eye(P))
+    prior_phi0, post_phi0 = rng.normal(size=C), rng.normal(size=C) * 3
+    keep = []
+
+    def D(a):
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+        keep.append(t)
+        return t
+
+    phi_d, pp_d, po_d = D(phi0.copy()), D(prior_phi0.copy()), D(post_phi0.copy())
+    xi_d, cand_d = torch.empty_like(phi_d), torch.empty_like(phi_d)
+    index = torch.empty(2 * C, dtype=torch.int32, device=cuda)
+    count = torch.zeros(1, dtype=torch.int32, device=cuda)
+    counters = torch.zeros(3, dtype=torch.int64, device=cuda)
+    cpost_s_d = torch.empty(C, dtype=torch.float64, device=cuda)
+    phi, prior_phi, post_phi = phi0.copy(), prior_phi0.copy(), post_phi0.copy()
+    moved = np.zeros(C, bool)
+    n_acc = n_rej = n_unl = 0
+    for sub in range(3):
+        step2 = step * _lib.TRAJ_MAX_SUBSTEPS + sub
+        xi = oracle.normals(seed, stage, step2, _lib.TAG_TRAJ_PROPOSAL, C, P)
+        _lib.call("altar_da_trajectory_propose", _lib.ptr(phi_d), C, P, _lib.ptr(D(L)), seed, stage, step, sub,
+                  _lib.ptr(xi_d), _lib.ptr(cand_d), None)
+        # normals: device log/sincos against the host's (last bits); the
+        # displacement from the same normals is bit for bit
+        xi_dev = xi_d.cpu().numpy()
+        assert np.max(np.abs(xi_dev - xi)) < 1e-12
+        cand = oracle.mh_propose(phi, L, xi_dev)
+        assert np.array_equal(cand_d.cpu().numpy(), cand)
+        cprior, cdata_s = rng.normal(size=C), rng.normal(size=C) * 3
+        mask = (rng.random(C) < 0.1).astype(np.int32)
+        cpost_s = cprior + beta * cdata_s
+        u = oracle.uniforms(seed, stage, step2, _lib.TAG_TRAJ_ACCEPT, C)
+        acc = (mask == 0) & (np.log(u) <= cpost_s - post_phi)
+        _lib.call("altar_da_trajectory_step", _lib.ptr(phi_d), _lib.ptr(pp_d), _lib.ptr(po_d), _lib.ptr(cand_d),
+                  _lib.ptr(D(cprior)), _lib.ptr(D(cdata_s)), _lib.ptr(cpost_s_d), _lib.ptr(D(mask)), beta, C, P,
+                  seed, stage, step, sub, _lib.ptr(index), _lib.ptr(counters), None)
+        phi[acc], prior_phi[acc], post_phi[acc] = cand[acc], cprior[acc], cpost_s[acc]
+        moved |= acc
+        n_acc += acc.sum()
+        n_rej += mask.sum()
+        n_unl += ((mask == 0) & ~acc).sum()
+        assert np.array_equal(cpost_s_d.cpu().numpy(), cpost_s)
+        assert np.array_equal(phi_d.cpu().numpy(), phi)
+        assert np.array_equal(pp_d.cpu().numpy(), prior_phi) and np.array_equal(po_d.cpu().numpy(), post_phi)
+    _lib.call("altar_da_trajectory_list", _lib.ptr(index), C, _lib.ptr(count), None)
+    n = int(count.item())
+    assert np.array_equal(index[:n].cpu().numpy(), np.nonzero(moved)[0])
+    assert counters.cpu().tolist() == [n_acc, n_rej, n_unl]
