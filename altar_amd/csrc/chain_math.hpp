@@ -1,5 +1,5 @@
-// Per-chain arithmetic shared by the separate sampler kernels and the fused
-// trajectory kernel.  Contraction into fused multiply-adds is switched off
+// Per-chain arithmetic and the network's activation, shared by the separate
+// sampler and network kernels and the fused trajectory kernel.  Contraction into fused multiply-adds is switched off
 // in these functions: every product and sum is rounded on its own, in the
 // order written, whoever inlines them (and as the FP64 oracle rounds them).
 #pragma once
@@ -39,11 +39,38 @@ __device__ inline double surrogate_llk_sum(const float *z, const double *e, cons
 {
 #pragma clang fp contract(off)
     double acc = cperp;
+#pragma unroll 8
     for (int k = 0; k < K; ++k) {
         const double t = e[k] + s[k] * (double)z[k];
         acc += t * t;
     }
     return norm - 0.5 * acc;
+}
+
+// GELU in its tanh form (Hendrycks and Gimpel 2016):
+//   gelu(x) = x/2 (1 + tanh(u)),  u = sqrt(2/pi) (x + 0.044715 x^3),
+// with tanh(u) = 1 - 2 / (1 + exp(2 u)).  Branch-free (exp and reciprocal
+// on the transcendental unit); exp(2u) overflows to +inf for large u, which
+// gives tanh = 1, and underflows to 0 for very negative u, tanh = -1.
+__device__ inline float gelu_tanh_t(float x)
+{
+#pragma clang fp contract(off)
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+}
+
+__device__ inline float gelu(float x)
+{
+#pragma clang fp contract(off)
+    return 0.5f * x * (1.0f + gelu_tanh_t(x));
+}
+
+// d gelu / dx = (1 + t) / 2 + x/2 (1 - t^2) sqrt(2/pi) (1 + 3 * 0.044715 x^2)
+__device__ inline float gelu_grad(float x)
+{
+#pragma clang fp contract(off)
+    const float t = gelu_tanh_t(x);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 0.134145f * x * x);
 }
 
 }  // namespace altar

@@ -23,14 +23,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int MT = 64, NT = 64, KT = 32;
 constexpr int MLDS = KT + 8;  // row pitch (bf16) of the LDS tiles
 
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-
-__device__ inline float gelu_grad(float x)
-{
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
-}
+// gelu and gelu_grad (tanh form): chain_math.hpp
 
 __device__ inline float bf2f(__bf16 v) { return (float)v; }
 __device__ inline __bf16 f2bf(float v) { return (__bf16)v; }

@@ -31,88 +31,68 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int TT = 64;             // chains per workgroup
-constexpr int THREADS = 256;       // 4 wavefronts
+constexpr int THREADS = 512;       // 8 wavefronts, two per SIMD
 constexpr int HID = 256;           // hidden width
 constexpr int KIN = 64;            // padded input width
 constexpr int KOUT = 64;           // padded output width
 constexpr int APITCH = HID + 8;    // bf16 row pitch of the activation tile
 constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
 
-__device__ inline float gelu_t(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-
 // One dense layer on the 64-row tile: act (64 x DIN bf16) times W^T
 // (W: DOUT x DIN bf16, row-major) plus bias; GELU into act (hidden layers,
 // after every wave has read it) or FP32 into zt (the last layer).  For
-// DOUT = 256 a wave computes 64 rows x 64 columns (2 x 2 MFMA tiles), for
-// DOUT = 64 one 32 x 32 tile.  The k order is mlp_gemm_kernel's: chunks of
-// 32, within a chunk the lanes' k-sets half * 16 + kk * 8 for kk = 0, 1.
+// DOUT = 256 each of the 8 waves computes 64 rows x 32 columns (2 MFMA
+// tiles), for DOUT = 64 waves 0-3 one 32 x 32 tile each.  The k order is
+// mlp_gemm_kernel's: chunks of 32, within a chunk the lanes' k-sets
+// half * 16 + kk * 8 for kk = 0, 1.
 template <int DIN, int DOUT, bool LAST>
 __device__ void tile_layer(__bf16 *act, float *zt, const __bf16 *__restrict__ W, const float *__restrict__ bias,
                            int lane, int wid)
 {
-    constexpr int RT = DOUT == HID ? 2 : 1, CT = DOUT == HID ? 2 : 1;
+    constexpr int RT = DOUT == HID ? 2 : 1;
     const int half = lane >> 5, l32 = lane & 31;
-    const int row0 = DOUT == HID ? 0 : 32 * (wid >> 1);
-    const int col0 = DOUT == HID ? 64 * wid : 32 * (wid & 1);
-    floatx16 acc[RT][CT];
+    const bool active = DOUT == HID || wid < 4;
+    const int row0 = DOUT == HID ? 0 : 32 * ((wid >> 1) & 1);
+    const int col0 = DOUT == HID ? 32 * wid : 32 * (wid & 1);
+    floatx16 acc[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
+        for (int v = 0; v < 16; ++v) acc[rt][v] = 0.0f;
+    // all of the wave's weight fragments of this layer in flight at once
+    // (DIN / 16 16-byte loads, at most 64 registers), then the MFMA chain
+    constexpr int KS = DIN / 16;
+    if (active) {
+        const __bf16 *wrow = W + (size_t)(col0 + l32) * DIN + half * 16;
+        bf16x8 b[KS];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[rt][ct][v] = 0.0f;
-    const __bf16 *wrow[CT];
+        for (int ks = 0; ks < KS; ++ks) b[ks] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) wrow[ct] = W + (size_t)(col0 + ct * 32 + l32) * DIN + half * 16;
-    // weights one chunk ahead in registers
-    bf16x8 bn[2][CT];
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) bn[kk][ct] = *(const bf16x8 *)(wrow[ct] + kk * 8);
-#pragma unroll
-    for (int k0 = 0; k0 < DIN; k0 += 32) {
-        bf16x8 b[2][CT];
-#pragma unroll
-        for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct) b[kk][ct] = bn[kk][ct];
-        if (k0 + 32 < DIN) {
-#pragma unroll
-            for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct) bn[kk][ct] = *(const bf16x8 *)(wrow[ct] + k0 + 32 + kk * 8);
-        }
-#pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
             bf16x8 a[RT];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-                a[rt] = *(const bf16x8 *)(act + (row0 + rt * 32 + l32) * APITCH + k0 + half * 16 + kk * 8);
+            for (int rt = 0; rt < RT; ++rt) a[rt] = *(const bf16x8 *)(act + (row0 + rt * 32 + l32) * APITCH + k);
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < CT; ++ct)
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[kk][ct], acc[rt][ct], 0, 0, 0);
+            for (int rt = 0; rt < RT; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[ks], acc[rt], 0, 0, 0);
         }
     }
     __syncthreads();  // every wave has read act
+    if (active) {
+        const int col = col0 + l32;
+        const float bc = bias[col];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct) {
-            const int col = col0 + ct * 32 + l32;
-            const float bc = bias[col];
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int row = row0 + rt * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
-                const float y = acc[rt][ct][v] + bc;
+                const float y = acc[rt][v] + bc;
                 if constexpr (LAST)
                     zt[row * (KOUT + 1) + col] = y;
                 else
-                    act[row * APITCH + col] = (__bf16)gelu_t(y);
+                    act[row * APITCH + col] = (__bf16)gelu(y);
             }
-        }
+    }
     __syncthreads();
 }
 

@@ -1,20 +1,25 @@
 """FP64 numpy restatement of the surrogate network (test infrastructure only).
 
-Dense layers Y = act(X W^T + b), GELU (erf form) on hidden layers, inputs
+Dense layers Y = act(X W^T + b), GELU (tanh form, Hendrycks and Gimpel
+2016) on hidden layers, inputs
 scaled to [-1, 1] by the prior bounds, loss sum_b sum_k w_k (y_bk - t_bk)^2 / B.
 Forward and backward by hand (chain rule), used to check the GPU kernels'
 gradients (altar_amd/csrc/mlp.hip).
 """
 import numpy as np
-from scipy.special import erf
+
+K0 = np.sqrt(2.0 / np.pi)
+K1 = 0.044715
 
 
 def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    """x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(K0 * (x + K1 * x ** 3)))
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(K0 * (x + K1 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * K0 * (1.0 + 3.0 * K1 * x * x)
 
 
 def scale_inputs(theta, lo, hi):
