@@ -75,11 +75,13 @@ def parse():
     ap.add_argument("--joint-da", action="store_true",
                     help="joint workload: also train the surrogate and time delayed acceptance against plain steps")
     ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
-    ap.add_argument("--da-substeps", default="16",
+    ap.add_argument("--da-substeps", default="128",
                     help="surrogate sub-steps per exact step in the trajectory leg, or a comma-separated "
                          "list (the first is the reported one; 1 disables the leg)")
-    ap.add_argument("--da-substep-scale", type=float, default=1.0,
+    ap.add_argument("--da-substep-scale", type=float, default=1.6,
                     help="sub-step proposal scale relative to the Metropolis proposal")
+    ap.add_argument("--da-moment-steps", default="300,20",
+                    help="plain,trajectory steps of the posterior-moment comparison (0,0 skips it)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -367,7 +369,7 @@ def side_workload(args):
         # 3 steps per stage, 196,608 training fields (3 exact steps of 65,536 chains)
         out["surrogate_da"] = da_leg(args, model, C, seed=777 + 5, capacity=3 * C, stage_steps=3,
                                      fit_kwargs=dict(epochs=20, batch=4096, lr=2e-3, pca_sample=16384),
-                                     validate_chains=512)
+                                     validate_chains=512, moments=False)
     if args.workload == "mogi":
         # the whole anneal from a fresh sample
         ann2 = Annealer(Metropolis(steps=5), COV(), seed=job.seed).initialize(job)
@@ -398,7 +400,72 @@ class JumpMonitor:
             self.steps += 1
 
 
-def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024):
+def fused_kernel_timing(ann, da, start, subs, reps=3):
+    """HIP-event time of one altar_da_trajectory_fused launch (all chains,
+    ``subs`` sub-steps) on the launch stream, and the network's algorithmic
+    rate: 2 (P 256 + 2 * 256 * 256 + 256 * 64) flops per chain and
+    evaluation, subs + 1 evaluations per launch."""
+    import torch
+    from altar_amd import _lib
+    model, sur = ann.model, da.surrogate
+    C, P = start.theta.shape
+    if not da.fused_supported(model, P):
+        return None
+    ws = da._workspace(C, P, start.theta.device)
+    cnt = torch.zeros(3, dtype=torch.int64, device=start.theta.device)
+    L = (da.sigma_chol * da.substep_scale).contiguous()
+    p = _lib.ptr
+
+    def launch(k):
+        _lib.call("altar_da_trajectory_fused", sur.mlp.handle, model._prior.handle, model.source.handle,
+                  p(start.theta), p(start.prior), p(ws["post_s"]), p(ws["phi"]), p(ws["prior_phi"]),
+                  p(ws["post_phi"]), p(L), p(sur.e_d), p(sur.s_d), sur.K, float(sur.cperp),
+                  float(model.normalization), float(start.beta), C, P, 99, 0, k, subs, p(ws["index"]), p(cnt),
+                  _lib.stream())
+    launch(0)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for k in range(reps):
+        launch(k + 1)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    flops = 2.0 * (P * 256 + 2 * 256 * 256 + 256 * 64) * C * (subs + 1)
+    return {"ms_per_launch": ms, "us_per_substep": 1e3 * ms / subs, "network_tflops": flops / (ms * 1e-3) / 1e12,
+            "network_flops_per_launch": flops, "substeps": subs}
+
+
+def moment_check(ann, plain, da, start, plain_steps, traj_steps, subs):
+    """Posterior moments at the same temperature from the same state: plain
+    Metropolis run long enough to decorrelate (plain_steps) against
+    trajectory delayed acceptance (traj_steps).  z_k = difference of the
+    means of parameter k over its Monte-Carlo standard error
+    sqrt(s_plain^2 / C + s_traj^2 / C)."""
+    import torch
+    C = start.theta.shape[0]
+    st_p = start.clone()
+    ann.sampler = plain
+    keep = plain.steps
+    plain.steps = plain_steps
+    plain.walkChains(ann, st_p)
+    plain.steps = keep
+    st_t = start.clone()
+    ann.sampler = da
+    keep = (da.steps, da.substeps)
+    da.steps, da.substeps = traj_steps, subs
+    da._walk_trajectory(ann, st_t)
+    da.steps, da.substeps = keep
+    torch.cuda.synchronize()
+    mp, mt = st_p.theta.mean(0), st_t.theta.mean(0)
+    sp, s_t = st_p.theta.std(0), st_t.theta.std(0)
+    z = ((mt - mp) / torch.sqrt((sp ** 2 + s_t ** 2) / C)).abs()
+    return {"plain_steps": plain_steps, "trajectory_steps": traj_steps,
+            "mean_z": z.tolist(), "max_mean_z": float(z.max()), "median_mean_z": float(z.median()),
+            "sd_ratio": (s_t / sp).tolist(), "max_sd_ratio_deviation": float((s_t / sp - 1).abs().max())}
+
+
+def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024,
+           moments=True):
     """Anneal with field collection until beta >= 0.1, fit the surrogate
     (PCA of whitened residuals + BF16 MLP), then time three samplers from the
     same state at the next temperature: plain Metropolis, two-stage delayed
@@ -503,6 +570,11 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
             runs.append(r)
         if runs:
             res["trajectory"] = runs[0]
+            subs = sweep[0]
+            res["trajectory"]["fused_kernel"] = fused_kernel_timing(ann, da, start, subs)
+            mp, mt = (int(v) for v in str(args.da_moment_steps).split(","))
+            if moments and mp > 0 and mt > 0:
+                res["moments"] = moment_check(ann, plain, da, start, mp, mt, subs)
         if len(runs) > 1:
             res["trajectory_sweep"] = runs
     ann.sampler = da
