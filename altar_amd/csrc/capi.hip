@@ -1221,6 +1221,7 @@ struct altar_mlp {
     std::vector<DevBuf> W32, Wm, Wv, W16, WT16, b32, bm, bv, gW, gb;
     std::vector<DevBuf> pre, act16, actT16, dY16, dYT16;
     DevBuf x16, xT16, out32, lo, hi, loss, tgt, wk;
+    DevBuf gpart, cpart;                    // split-k and column-sum partials of the backward pass
     long step = 0;
     int P = 0;
 };
@@ -1251,6 +1252,17 @@ void mlp_forward(altar_mlp *h, const double *theta, int B, hipStream_t st)
     }
 }
 
+// k-ranges of the weight-gradient GEMM over a batch of B rows (B a multiple
+// of 64): the most slices, up to 16, that keep each range >= 128 rows and a
+// multiple of the 32-row k tile
+constexpr int SPLITK_MAX = 16;
+int splitk_slices(int B)
+{
+    for (int S = SPLITK_MAX; S > 1; S /= 2)
+        if (B % (S * 32) == 0 && B / S >= 128) return S;
+    return 1;
+}
+
 void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int valid, hipStream_t st)
 {
     const int L = h->nlayers;
@@ -1264,11 +1276,10 @@ void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int 
         const void *X = l == 0 ? h->x16.p : h->act16[l - 1].p;
         void *XT = l == 0 ? h->xT16.p : h->actT16[l - 1].p;
         hip_check(launch_transpose_bf16(X, B, K, K, XT, B, st), "transpose");
-        MlpEpi ep{};
-        ep.out32 = h->gW[l].as<float>();
-        ep.ldo = K;
-        hip_check(launch_mlp_gemm(MLP_EPI_F32, h->dYT16[l].p, B, XT, B, O, K, B, ep, st), "mlp dW");
-        hip_check(launch_colsum(h->dY16[l].p, B, O, O, h->gb[l].as<float>(), st), "mlp db");
+        hip_check(launch_mlp_gemm_splitk(h->dYT16[l].p, B, XT, B, O, K, B, splitk_slices(B), h->gpart.as<float>(),
+                                         h->gW[l].as<float>(), st),
+                  "mlp dW");
+        hip_check(launch_colsum(h->dY16[l].p, B, O, O, h->cpart.as<float>(), h->gb[l].as<float>(), st), "mlp db");
         if (l > 0) {
             // data gradient through the previous GELU: dX = (dY . W) * gelu'(pre)
             MlpEpi ed{};
@@ -1322,6 +1333,13 @@ int altar_mlp_create(altar_mlp **out, int nin, int nhid, int nlayers, int nout, 
         h->loss.reserve(4, true);
         h->tgt.reserve(B * h->kout * 4, true);
         h->wk.reserve(h->kout * 4, true);
+        size_t maxOK = 0, maxO = 0;
+        for (int l = 0; l < L; ++l) {
+            maxOK = std::max(maxOK, (size_t)h->dout[l] * h->din[l]);
+            maxO = std::max(maxO, (size_t)h->dout[l]);
+        }
+        h->gpart.reserve((size_t)SPLITK_MAX * maxOK * 4, true);
+        h->cpart.reserve((size_t)colsum_slabs((int)B) * maxO * 4, true);
         *out = h;
     });
 }
@@ -1333,7 +1351,9 @@ int altar_mlp_destroy(altar_mlp *h)
         for (auto *v : {&h->W32, &h->Wm, &h->Wv, &h->W16, &h->WT16, &h->b32, &h->bm, &h->bv, &h->gW, &h->gb,
                         &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16})
             for (auto &b : *v) b.release();
-        for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk}) b->release();
+        for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk, &h->gpart,
+                          &h->cpart})
+            b->release();
         delete h;
     });
 }

@@ -83,6 +83,7 @@ struct MlpEpi {
     __bf16 *out16;
     int ldo;
     int act;              // BIAS_ACT: apply GELU
+    size_t split_stride;  // F32 with split-k: slice z is written at out32 + z * split_stride
 };
 
 struct AcceptArgs {
@@ -144,7 +145,12 @@ hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int l
 hipError_t launch_transpose_bf16(const void *in, int rows, int cols, int ldi, void *out, int ldo, hipStream_t st);
 hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, int rows, int cols, int ld,
                            int valid_rows, void *dy, float *loss, hipStream_t st);
-hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *db, hipStream_t st);
+// db = column sums of dy (rows x cols); part: colsum_slabs(rows) x cols scratch
+int colsum_slabs(int rows);
+hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *part, float *db, hipStream_t st);
+// out (M x N FP32) = A B^T over K split into S ranges (K % (32 S) == 0); part: S x M x N scratch
+hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb, int M, int N, int K, int S,
+                                  float *part, float *out, hipStream_t st);
 hipError_t launch_adam(float *w, float *m, float *v, const float *g, int rows, int cols, float lr, float b1,
                        float b2, float eps, float c1, float c2, void *w16, void *wt16, hipStream_t st);
 hipError_t launch_surrogate_llk(const float *z, int ldz, int C, int K, const double *e, const double *s,

@@ -44,7 +44,10 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict_
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
     const int half = lane >> 5, l32 = lane & 31;
-    for (int k0 = 0; k0 < K; k0 += KT) {
+    // split-k: block z covers k in [z K / gridDim.z, (z + 1) K / gridDim.z)
+    const int kspan = K / (int)gridDim.z;
+    const int kbeg = (int)blockIdx.z * kspan;
+    for (int k0 = kbeg; k0 < kbeg + kspan; k0 += KT) {
         *(bf16x8 *)(&As[lr * MLDS + lc]) = *(const bf16x8 *)(A + (size_t)(m0 + lr) * lda + k0 + lc);
         *(bf16x8 *)(&Bs[lr * MLDS + lc]) = *(const bf16x8 *)(B + (size_t)(n0 + lr) * ldb + k0 + lc);
         __syncthreads();
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict_
             ep.out16[(size_t)row * ep.ldo + col] = f2bf(o);
             if (ep.out32) ep.out32[(size_t)row * ep.ldo + col] = o;
         } else if constexpr (EPI == MLP_EPI_F32) {
-            ep.out32[(size_t)row * ep.ldo + col] = y;
+            ep.out32[(size_t)blockIdx.z * ep.split_stride + (size_t)row * ep.ldo + col] = y;
         } else {  // MLP_EPI_DGELU
             float g = y * gelu_grad(ep.pre[(size_t)row * ep.ldo + col]);
             ep.out16[(size_t)row * ep.ldo + col] = f2bf(g);
@@ -102,37 +105,70 @@ __global__ void mse_grad_kernel(const float *__restrict__ yhat, const float *__r
                                 float *__restrict__ loss)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)rows * cols) return;
-    const int r = (int)(idx / cols), c = (int)(idx % cols);
     float g = 0.0f, l = 0.0f;
-    if (r < valid_rows) {
-        const float e = yhat[(size_t)r * ld + c] - y[(size_t)r * ld + c];
-        g = 2.0f * w[c] * e / (float)valid_rows;
-        l = w[c] * e * e / (float)valid_rows;
+    if (idx < (long)rows * cols) {
+        const int r = (int)(idx / cols), c = (int)(idx % cols);
+        if (r < valid_rows) {
+            const float e = yhat[(size_t)r * ld + c] - y[(size_t)r * ld + c];
+            g = 2.0f * w[c] * e / (float)valid_rows;
+            l = w[c] * e * e / (float)valid_rows;
+        }
+        dy[(size_t)r * ld + c] = f2bf(g);
     }
-    dy[(size_t)r * ld + c] = f2bf(g);
-    if (loss && l != 0.0f) atomicAdd(loss, l);
+    // one atomic per block (the loss is for monitoring only)
+    __shared__ float part[4];
+    l = wave_sum(l);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0 && loss) {
+        const float t = part[0] + part[1] + part[2] + part[3];
+        if (t != 0.0f) atomicAdd(loss, t);
+    }
 }
 
-// db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient).  A block owns 64
-// columns (one per lane, so every row read is coalesced); its 16 wavefronts
-// split the rows and their partial sums are added in a fixed order.
-__global__ __launch_bounds__(1024) void colsum_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld,
-                                                      float *__restrict__ db)
+// db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient), in two passes.
+// Pass 1: block (x, y) sums rows [256 y, 256 y + 256) of columns
+// [64 x, 64 x + 64): lane = column (coalesced rows), 4 waves of 64 rows each,
+// wave partials added in order into part[y][column].  Pass 2 adds the slabs
+// in order, so the result does not depend on scheduling.
+constexpr int CS_ROWS = 256;
+__global__ __launch_bounds__(256) void colsum_slab_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld,
+                                                          float *__restrict__ part)
 {
-    __shared__ float part[16][64];
+    __shared__ float wp[4][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + lane;
+    const int r0 = blockIdx.y * CS_ROWS + wid * 64;
     float s = 0.0f;
-    if (c < cols)
-        for (int r = wid; r < rows; r += 16) s += bf2f(dy[(size_t)r * ld + c]);
-    part[wid][lane] = s;
-    __syncthreads();
-    if (wid == 0 && c < cols) {
-        float t = 0.0f;
-        for (int w = 0; w < 16; ++w) t += part[w][lane];
-        db[c] = t;
+    if (c < cols) {
+#pragma unroll 8
+        for (int i = 0; i < 64; ++i) {
+            const int r = r0 + i;
+            if (r < rows) s += bf2f(dy[(size_t)r * ld + c]);
+        }
     }
+    wp[wid][lane] = s;
+    __syncthreads();
+    if (wid == 0 && c < cols) part[(size_t)blockIdx.y * cols + c] = ((wp[0][lane] + wp[1][lane]) + wp[2][lane]) + wp[3][lane];
+}
+
+__global__ void colsum_total_kernel(const float *__restrict__ part, int slabs, int cols, float *__restrict__ db)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float t = 0.0f;
+    for (int y = 0; y < slabs; ++y) t += part[(size_t)y * cols + c];
+    db[c] = t;
+}
+
+// out[i] = sum_z part[z stride + i], z in order (split-k partial products)
+__global__ void splitk_sum_kernel(const float *__restrict__ part, int S, size_t stride, long n, float *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = 0.0f;
+    for (int z = 0; z < S; ++z) t += part[(size_t)z * stride + i];
+    out[i] = t;
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy and
@@ -183,7 +219,7 @@ __global__ void mlp_input_kernel(const double *__restrict__ theta, int C, int P,
 hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
                            const MlpEpi &ep, hipStream_t st)
 {
-    dim3 grid(N / NT, M / MT), block(256);
+    dim3 grid(N / NT, M / MT, 1), block(256);
     const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
     if (epi == MLP_EPI_BIAS_ACT)
         hipLaunchKernelGGL((mlp_gemm_kernel<MLP_EPI_BIAS_ACT>), grid, block, 0, st, a, lda, b, ldb, K, ep);
@@ -211,10 +247,30 @@ hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, in
     return hipGetLastError();
 }
 
-hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *db, hipStream_t st)
+int colsum_slabs(int rows) { return (rows + CS_ROWS - 1) / CS_ROWS; }
+
+hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *part, float *db, hipStream_t st)
 {
-    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(1024), 0, st, (const __bf16 *)dy, rows, cols, ld,
-                       db);
+    const int slabs = colsum_slabs(rows);
+    hipLaunchKernelGGL(colsum_slab_kernel, dim3((cols + 63) / 64, slabs), dim3(256), 0, st, (const __bf16 *)dy, rows,
+                       cols, ld, part);
+    hipLaunchKernelGGL(colsum_total_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, part, slabs, cols, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb, int M, int N, int K, int S,
+                                  float *part, float *out, hipStream_t st)
+{
+    MlpEpi ep{};
+    ep.out32 = part;
+    ep.ldo = N;
+    ep.split_stride = (size_t)M * N;
+    dim3 grid(N / NT, M / MT, S), block(256);
+    hipLaunchKernelGGL((mlp_gemm_kernel<MLP_EPI_F32>), grid, block, 0, st, (const __bf16 *)A, lda, (const __bf16 *)B,
+                       ldb, K, ep);
+    const long n = (long)M * N;
+    hipLaunchKernelGGL(splitk_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, S,
+                       ep.split_stride, n, out);
     return hipGetLastError();
 }
 

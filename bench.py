@@ -80,7 +80,7 @@ def parse():
                          "list (the first is the reported one; 1 disables the leg)")
     ap.add_argument("--da-substep-scale", type=float, default=1.6,
                     help="sub-step proposal scale relative to the Metropolis proposal")
-    ap.add_argument("--da-moment-steps", default="300,20",
+    ap.add_argument("--da-moment-steps", default="1000,20",
                     help="plain,trajectory steps of the posterior-moment comparison (0,0 skips it)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
