@@ -435,33 +435,50 @@ def fused_kernel_timing(ann, da, start, subs, reps=3):
             "network_flops_per_launch": flops, "substeps": subs}
 
 
-def moment_check(ann, plain, da, start, plain_steps, traj_steps, subs):
+def moment_check(ann, plain, da, start, plain_steps, traj_steps, subs, reps=4):
     """Posterior moments at the same temperature from the same state: plain
     Metropolis run long enough to decorrelate (plain_steps) against
-    trajectory delayed acceptance (traj_steps).  z_k = difference of the
-    means of parameter k over its Monte-Carlo standard error
-    sqrt(s_plain^2 / C + s_traj^2 / C)."""
+    trajectory delayed acceptance (traj_steps), each repeated ``reps`` times
+    on independent random streams (other seeds).  The chains of one run are
+    not independent (resampling copies, slow mixing), so the Monte-Carlo
+    standard error of a sampler's mean is the spread of its replicate means:
+    se = sd_r(mean) / sqrt(reps).  z_k = (mean_traj - mean_plain) / sqrt(se_traj^2 + se_plain^2)
+    for parameter k; ``naive_se_factor`` is se_plain over sd / sqrt(C)."""
     import torch
     C = start.theta.shape[0]
-    st_p = start.clone()
-    ann.sampler = plain
-    keep = plain.steps
-    plain.steps = plain_steps
-    plain.walkChains(ann, st_p)
-    plain.steps = keep
-    st_t = start.clone()
-    ann.sampler = da
-    keep = (da.steps, da.substeps)
-    da.steps, da.substeps = traj_steps, subs
-    da._walk_trajectory(ann, st_t)
-    da.steps, da.substeps = keep
+    seed0 = ann.seed
+    means = {"plain": [], "trajectory": []}
+    sds = {"plain": [], "trajectory": []}
+    keep_p, keep_t = plain.steps, (da.steps, da.substeps)
+    try:
+        for r in range(reps):
+            ann.seed = seed0 + 7919 * (r + 1)
+            st = start.clone()
+            ann.sampler = plain
+            plain.steps = plain_steps
+            plain.walkChains(ann, st)
+            means["plain"].append(st.theta.mean(0))
+            sds["plain"].append(st.theta.std(0))
+            st = start.clone()
+            ann.sampler = da
+            da.steps, da.substeps = traj_steps, subs
+            da._walk_trajectory(ann, st)
+            means["trajectory"].append(st.theta.mean(0))
+            sds["trajectory"].append(st.theta.std(0))
+    finally:
+        ann.seed = seed0
+        plain.steps = keep_p
+        da.steps, da.substeps = keep_t
     torch.cuda.synchronize()
-    mp, mt = st_p.theta.mean(0), st_t.theta.mean(0)
-    sp, s_t = st_p.theta.std(0), st_t.theta.std(0)
-    z = ((mt - mp) / torch.sqrt((sp ** 2 + s_t ** 2) / C)).abs()
-    return {"plain_steps": plain_steps, "trajectory_steps": traj_steps,
-            "mean_z": z.tolist(), "max_mean_z": float(z.max()), "median_mean_z": float(z.median()),
-            "sd_ratio": (s_t / sp).tolist(), "max_sd_ratio_deviation": float((s_t / sp - 1).abs().max())}
+    M = {k: torch.stack(v) for k, v in means.items()}
+    S = {k: torch.stack(v).mean(0) for k, v in sds.items()}
+    se = {k: M[k].std(0) / reps ** 0.5 for k in M}
+    z = (M["trajectory"].mean(0) - M["plain"].mean(0)) / torch.sqrt(se["trajectory"] ** 2 + se["plain"] ** 2)
+    ratio = S["trajectory"] / S["plain"]
+    return {"plain_steps": plain_steps, "trajectory_steps": traj_steps, "replicates": reps,
+            "mean_z": z.tolist(), "max_abs_mean_z": float(z.abs().max()), "median_abs_mean_z": float(z.abs().median()),
+            "sd_ratio": ratio.tolist(), "max_sd_ratio_deviation": float((ratio - 1).abs().max()),
+            "naive_se_factor": (se["plain"] / (S["plain"] / C ** 0.5)).tolist()}
 
 
 def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024,
