@@ -44,7 +44,7 @@ from .metropolis import Metropolis
 
 class DelayedAcceptance(Metropolis):
     def __init__(self, surrogate, train_beta=0.1, max_rmse=2e-3, fit_kwargs=None, validate_chains=1024,
-                 retrain=True, substeps=1, substep_scale=1.0, fused=True, **kwds):
+                 retrain=True, substeps=1, substep_scale=1.0, substep_target=None, fused=True, **kwds):
         super().__init__(**kwds)
         if not 1 <= substeps <= _lib.TRAJ_MAX_SUBSTEPS:
             raise ValueError(f"substeps must lie in [1, {_lib.TRAJ_MAX_SUBSTEPS}]")
@@ -52,6 +52,9 @@ class DelayedAcceptance(Metropolis):
         self.substeps = substeps
         self.substep_scale = substep_scale   # sub-step proposal = substep_scale * (L xi)
         self.fused = fused                   # whole trajectories in one launch where supported
+        # with a target sub-step acceptance rate, each stage first tunes
+        # substep_scale by pilot trajectories (which leave the chains as they are)
+        self.substep_target = substep_target
         self.substep_counts = [0, 0, 0]   # surrogate sub-steps: accepted, rejected by the model, unlikely
         self.retrain = retrain
         self._refit = False
@@ -205,6 +208,37 @@ class DelayedAcceptance(Metropolis):
                     and 2 <= m.nlayers <= 8 and getattr(model, "offset", 0) == 0 and model.parameters == P
                     and hasattr(model, "_prior") and hasattr(model, "source"))
 
+    PILOT_STEP = 1 << 21     # step index of the pilot trajectories' streams (the walk's steps stay below)
+
+    def tune_substep_scale(self, annealer, step, ws, rounds=4, substeps=16, tolerance=0.05):
+        """Scale the sub-step proposal so that the surrogate sub-steps accept
+        about ``substep_target`` of their proposals.  Each round runs one
+        fused trajectory of ``substeps`` sub-steps from the current states into
+        the workspace (the chains do not move) and multiplies the scale by
+        sqrt(rate / target), within [1/2, 2].  The scale is a function of the
+        whole population, like the proposal covariance, and stays fixed
+        during the walk."""
+        model, sur = annealer.model, self.surrogate
+        C, P = step.samples, step.parameters
+        p = _lib.ptr
+        cnt = torch.zeros(3, dtype=torch.int64, device=step.theta.device)
+        target = float(self.substep_target)
+        for r in range(rounds):
+            L = (self.sigma_chol * self.substep_scale).contiguous()
+            cnt.zero_()
+            _lib.call("altar_da_trajectory_fused", sur.mlp.handle, model._prior.handle, model.source.handle,
+                      p(step.theta), p(step.prior), p(ws["post_s"]), p(ws["phi"]), p(ws["prior_phi"]),
+                      p(ws["post_phi"]), p(L), p(sur.e_d), p(sur.s_d), sur.K, float(sur.cperp),
+                      float(model.normalization), float(step.beta), C, P, annealer.seed,
+                      annealer.worker.iteration, self.PILOT_STEP + r, substeps, p(ws["index"]), p(cnt),
+                      _lib.stream())
+            a, rj, u = (int(v) for v in cnt.cpu().tolist())
+            rate = a / max(a + rj + u, 1)
+            if abs(rate - target) < tolerance:
+                break
+            self.substep_scale *= float(np.clip(np.sqrt(max(rate, 1e-3) / target), 0.5, 2.0))
+        return self.substep_scale
+
     def _walk_trajectory(self, annealer, step):
         """Exact steps along surrogate trajectories of ``substeps`` sub-steps.
 
@@ -231,6 +265,8 @@ class DelayedAcceptance(Metropolis):
         counters = torch.zeros(4, dtype=torch.int64, device=dev)
         sub_counters = torch.zeros(3, dtype=torch.int64, device=dev)
         candidate = CoolingStep(beta, ws["cand"], (ws["cprior"], ws["cdata_s"], ws["cpost_s"]), sigma=step.sigma)
+        if fused and self.substep_target is not None:
+            self.tune_substep_scale(annealer, step, ws)
         L = self.sigma_chol if self.substep_scale == 1.0 else (self.sigma_chol * self.substep_scale).contiguous()
         d = annealer.dispatcher
         moved_total = 0

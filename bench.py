@@ -80,8 +80,10 @@ def parse():
                          "list (the first is the reported one; 1 disables the leg)")
     ap.add_argument("--da-substep-scale", type=float, default=1.6,
                     help="sub-step proposal scale relative to the Metropolis proposal")
-    ap.add_argument("--da-moment-steps", default="1000,20",
-                    help="plain,trajectory steps of the posterior-moment comparison (0,0 skips it)")
+    ap.add_argument("--da-moment-steps", type=int, default=50,
+                    help="exact steps per replicate of the posterior-moment comparison (0 skips it)")
+    ap.add_argument("--da-substep-target", type=float, default=0.3,
+                    help="sub-step acceptance rate the pilot trajectories tune the sub-step scale to (0: fixed)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -435,50 +437,66 @@ def fused_kernel_timing(ann, da, start, subs, reps=3):
             "network_flops_per_launch": flops, "substeps": subs}
 
 
-def moment_check(ann, plain, da, start, plain_steps, traj_steps, subs, reps=4):
-    """Posterior moments at the same temperature from the same state: plain
-    Metropolis run long enough to decorrelate (plain_steps) against
-    trajectory delayed acceptance (traj_steps), each repeated ``reps`` times
-    on independent random streams (other seeds).  The chains of one run are
-    not independent (resampling copies, slow mixing), so the Monte-Carlo
-    standard error of a sampler's mean is the spread of its replicate means:
-    se = sd_r(mean) / sqrt(reps).  z_k = (mean_traj - mean_plain) / sqrt(se_traj^2 + se_plain^2)
-    for parameter k; ``naive_se_factor`` is se_plain over sd / sqrt(C)."""
+def moment_check(ann, da, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equilibrate=100):
+    """Does the exact stage correct the surrogate's error?  Posterior means
+    at one temperature by trajectory delayed acceptance with the trained
+    surrogate and with a deliberately biased one (its PCA-coefficient offsets
+    e_k shifted by N(0, bias_sd^2) draws: a smooth error in the surrogate
+    llk).  Both must sample the same exact posterior.
+    * Common start: the trained sampler's population after ``equilibrate``
+      exact steps, so that neither run carries memory of a start that is
+      off the posterior (a biased surrogate forgets it more slowly).
+    * Three configurations of ``reps`` runs of ``steps`` exact steps on
+      independent random streams: trained, trained again (control, other
+      streams) and biased.
+    * z_k = difference of the replicate-averaged means of parameter k over
+      its standard error: ``z_naive`` with sd / sqrt(reps C) per sampler
+      (the chains have forgotten the start), ``z_rep`` with the replicate
+      spread.  ``chi2`` = sum_k z_naive^2 (10 parameters: 18.3 is the 95%
+      point for independent z), shown for the control beside the test."""
     import torch
+    sur = da.surrogate
     C = start.theta.shape[0]
-    seed0 = ann.seed
-    means = {"plain": [], "trajectory": []}
-    sds = {"plain": [], "trajectory": []}
-    keep_p, keep_t = plain.steps, (da.steps, da.substeps)
+    seed0, keep = ann.seed, (da.steps, da.substeps)
+    e_true = sur.e_d
+    shift = torch.from_numpy(np.random.default_rng(seed).normal(0.0, bias_sd, sur.K)).to(e_true.device)
+    res = {}
     try:
-        for r in range(reps):
-            ann.seed = seed0 + 7919 * (r + 1)
-            st = start.clone()
-            ann.sampler = plain
-            plain.steps = plain_steps
-            plain.walkChains(ann, st)
-            means["plain"].append(st.theta.mean(0))
-            sds["plain"].append(st.theta.std(0))
-            st = start.clone()
-            ann.sampler = da
-            da.steps, da.substeps = traj_steps, subs
-            da._walk_trajectory(ann, st)
-            means["trajectory"].append(st.theta.mean(0))
-            sds["trajectory"].append(st.theta.std(0))
+        ann.sampler = da
+        da.substeps = subs
+        base = start.clone()
+        ann.seed = seed0 + 1
+        da.steps = equilibrate
+        da._walk_trajectory(ann, base)
+        da.steps = steps
+        for i, (name, e) in enumerate((("trained", e_true), ("control", e_true),
+                                       ("biased", (e_true + shift).contiguous()))):
+            sur.e_d = e
+            means, sds, acc = [], [], 0
+            for r in range(reps):
+                ann.seed = seed0 + 7919 * (1 + r + i * reps)
+                st = base.clone()
+                a, _, _ = da._walk_trajectory(ann, st)
+                acc += a
+                means.append(st.theta.mean(0))
+                sds.append(st.theta.std(0))
+            res[name] = (torch.stack(means), torch.stack(sds).mean(0), acc / (reps * steps * C))
     finally:
+        sur.e_d = e_true
         ann.seed = seed0
-        plain.steps = keep_p
-        da.steps, da.substeps = keep_t
+        da.steps, da.substeps = keep
     torch.cuda.synchronize()
-    M = {k: torch.stack(v) for k, v in means.items()}
-    S = {k: torch.stack(v).mean(0) for k, v in sds.items()}
-    se = {k: M[k].std(0) / reps ** 0.5 for k in M}
-    z = (M["trajectory"].mean(0) - M["plain"].mean(0)) / torch.sqrt(se["trajectory"] ** 2 + se["plain"] ** 2)
-    ratio = S["trajectory"] / S["plain"]
-    return {"plain_steps": plain_steps, "trajectory_steps": traj_steps, "replicates": reps,
-            "mean_z": z.tolist(), "max_abs_mean_z": float(z.abs().max()), "median_abs_mean_z": float(z.abs().median()),
-            "sd_ratio": ratio.tolist(), "max_sd_ratio_deviation": float((ratio - 1).abs().max()),
-            "naive_se_factor": (se["plain"] / (S["plain"] / C ** 0.5)).tolist()}
+
+    def compare(a, b):
+        (Ma, sa, _), (Mb, sb, _) = res[a], res[b]
+        d = Ma.mean(0) - Mb.mean(0)
+        zn = d / torch.sqrt((sa ** 2 + sb ** 2) / (reps * C))
+        zr = d / torch.sqrt(Ma.var(0) / reps + Mb.var(0) / reps)
+        return {"z_naive": zn.tolist(), "z_rep": zr.tolist(), "chi2": float((zn ** 2).sum()),
+                "max_abs_z": float(zn.abs().max()), "sd_ratio_max_deviation": float((sa / sb - 1).abs().max())}
+    return {"equilibrate_steps": equilibrate, "steps": steps, "replicates": reps, "bias_sd": bias_sd,
+            "exact_acceptance": {k: v[2] for k, v in res.items()},
+            "trained_vs_biased": compare("trained", "biased"), "trained_vs_control": compare("trained", "control")}
 
 
 def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024,
@@ -538,6 +556,17 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     sweep = [max(1, int(v)) for v in str(args.da_substeps).split(",")]
     subs = sweep[0]
     da.substep_scale = args.da_substep_scale
+    tuning = None
+    if args.da_substep_target and ok and da.fused_supported(model, start.theta.shape[1]):
+        # the stage's pilot tuning of the sub-step scale, once, outside the
+        # timed runs (which then share the scale, so the replays repeat them)
+        da.substep_target = args.da_substep_target
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        da.tune_substep_scale(ann, start, da._workspace(C, start.theta.shape[1], start.theta.device))
+        torch.cuda.synchronize()
+        tuning = {"target": da.substep_target, "scale": da.substep_scale, "ms": 1e3 * (time.perf_counter() - t3)}
+    da.substep_target = None
 
     def walk(kind):
         if kind == "plain":
@@ -583,15 +612,15 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
             sub0 = list(da.substep_counts)
             r = run("trajectory")
             sc = [b - a for a, b in zip(sub0, da.substep_counts)]
-            r.update(substeps=n, substep_scale=da.substep_scale, substep_acceptance=sc[0] / max(sum(sc), 1))
+            r.update(substeps=n, substep_scale=da.substep_scale, substep_acceptance=sc[0] / max(sum(sc), 1),
+                     substep_tuning=tuning)
             runs.append(r)
         if runs:
             res["trajectory"] = runs[0]
             subs = sweep[0]
             res["trajectory"]["fused_kernel"] = fused_kernel_timing(ann, da, start, subs)
-            mp, mt = (int(v) for v in str(args.da_moment_steps).split(","))
-            if moments and mp > 0 and mt > 0:
-                res["moments"] = moment_check(ann, plain, da, start, mp, mt, subs)
+            if moments and args.da_moment_steps > 0:
+                res["moments"] = moment_check(ann, da, start, args.da_moment_steps, subs)
         if len(runs) > 1:
             res["trajectory_sweep"] = runs
     ann.sampler = da
@@ -608,10 +637,6 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
         if r is not None and pl["accepted"]:
             r["gain_accepted_per_s"] = r["accepted_samples_per_s"] / pl["accepted_samples_per_s"]
             r["gain_esjd_per_s"] = r["esjd_per_s"] / pl["esjd_per_s"] if pl["esjd_per_s"] else None
-            # final sample means against plain Metropolis's, in standard errors
-            ma, mp = np.array(r["final_means"]), np.array(pl["final_means"])
-            se = np.sqrt((np.array(r["final_sds"]) ** 2 + np.array(pl["final_sds"]) ** 2) / C)
-            r["max_mean_difference_se"] = float(np.max(np.abs(ma - mp) / se))
     for kind in ("da", "trajectory"):
         if kind in res and "gain_accepted_per_s" in res[kind]:
             out[f"{kind}_gain_accepted_per_s"] = res[kind]["gain_accepted_per_s"]

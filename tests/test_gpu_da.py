@@ -198,6 +198,29 @@ def test_trajectory_linear_gaussian_posterior(cuda, fused):
     assert sur.trained and "da" in modes and da.substep_counts[0] > 0
     assert np.all(np.abs(th.mean(0) - mu) < 5 * se)
     assert np.all(np.abs(th.std(0) / sd - 1) < 0.1)
+    # what the network gains: squared displacement (in posterior standard
+    # deviations) per exact evaluation, trajectories against plain Metropolis
+    # over 3 steps from the final state
+    ann.model = model
+    final = ann.worker.step
+    sdev = final.theta.std(0)
+
+    def jump(walk):
+        st = final.clone()
+        before = st.theta.clone()
+        walk(st)
+        return float((((st.theta - before) / sdev) ** 2).sum(1).mean())
+    plain = Metropolis(steps=3)
+    plain.scaling, plain.sigma_chol = da.scaling, da.sigma_chol
+    ann.sampler = plain
+    j_plain = jump(lambda st: plain.walkChains(ann, st)) / (3 * C)
+    ann.sampler = da
+    da.steps = 3
+    ev0 = da.exact_evals
+    j_traj = jump(lambda st: da._walk_trajectory(ann, st))
+    j_traj /= max(da.exact_evals - ev0, 1)
+    print("squared jump per exact evaluation: plain", j_plain, "trajectories", j_traj)
+    assert j_traj > 2.0 * j_plain
 
 
 def test_fused_trajectory_matches_separate_calls(cuda):
