@@ -33,18 +33,39 @@ __device__ inline double gauss_log_term(double t, double mean, double sd, double
 }
 
 // surrogate data llk from K network outputs z (unit stride):
-// norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2, summed over k in order
-__device__ inline double surrogate_llk_sum(const float *z, const double *e, const double *s, int K, double cperp,
-                                           double norm)
+// norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2.  The sum is taken as four
+// interleaved partial sums (k = q mod 4, each over k in order), added to
+// cperp in the order q = 0, 1, 2, 3, so that four threads can share a row.
+constexpr int SLLK_PARTS = 4;
+
+__device__ inline double surrogate_llk_part(const float *z, const double *e, const double *s, int K, int q)
 {
 #pragma clang fp contract(off)
-    double acc = cperp;
-#pragma unroll 8
-    for (int k = 0; k < K; ++k) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = q; k < K; k += SLLK_PARTS) {
         const double t = e[k] + s[k] * (double)z[k];
         acc += t * t;
     }
+    return acc;
+}
+
+__device__ inline double surrogate_llk_combine(double p0, double p1, double p2, double p3, double cperp, double norm)
+{
+#pragma clang fp contract(off)
+    double acc = cperp;
+    acc += p0;
+    acc += p1;
+    acc += p2;
+    acc += p3;
     return norm - 0.5 * acc;
+}
+
+__device__ inline double surrogate_llk_sum(const float *z, const double *e, const double *s, int K, double cperp,
+                                           double norm)
+{
+    return surrogate_llk_combine(surrogate_llk_part(z, e, s, K, 0), surrogate_llk_part(z, e, s, K, 1),
+                                 surrogate_llk_part(z, e, s, K, 2), surrogate_llk_part(z, e, s, K, 3), cperp, norm);
 }
 
 // GELU in its tanh form (Hendrycks and Gimpel 2016):

@@ -36,18 +36,20 @@ constexpr int HID = 256;           // hidden width
 constexpr int KIN = 64;            // padded input width
 constexpr int KOUT = 64;           // padded output width
 constexpr int APITCH = HID + 8;    // bf16 row pitch of the activation tile
+constexpr int XPITCH = KIN + 8;    // bf16 row pitch of the network-input tile
 constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
 
-// One dense layer on the 64-row tile: act (64 x DIN bf16) times W^T
-// (W: DOUT x DIN bf16, row-major) plus bias; GELU into act (hidden layers,
-// after every wave has read it) or FP32 into zt (the last layer).  For
+// One dense layer on the 64-row tile: in (64 x DIN bf16, row pitch ipitch)
+// times W^T (W: DOUT x DIN bf16, row-major) plus bias; GELU into act (hidden
+// layers, after every wave has read its input, which may be act itself) or
+// FP32 into zt (the last layer).  For
 // DOUT = 256 each of the 8 waves computes 64 rows x 32 columns (2 MFMA
 // tiles), for DOUT = 64 waves 0-3 one 32 x 32 tile each.  The k order is
 // mlp_gemm_kernel's: chunks of 32, within a chunk the lanes' k-sets
 // half * 16 + kk * 8 for kk = 0, 1.
 template <int DIN, int DOUT, bool LAST>
-__device__ void tile_layer(__bf16 *act, float *zt, const __bf16 *__restrict__ W, const float *__restrict__ bias,
-                           int lane, int wid)
+__device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt, const __bf16 *__restrict__ W,
+                           const float *__restrict__ bias, int lane, int wid)
 {
     constexpr int RT = DOUT == HID ? 2 : 1;
     const int half = lane >> 5, l32 = lane & 31;
@@ -72,12 +74,12 @@ __device__ void tile_layer(__bf16 *act, float *zt, const __bf16 *__restrict__ W,
             const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
             bf16x8 a[RT];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) a[rt] = *(const bf16x8 *)(act + (row0 + rt * 32 + l32) * APITCH + k);
+            for (int rt = 0; rt < RT; ++rt) a[rt] = *(const bf16x8 *)(in + (row0 + rt * 32 + l32) * ipitch + k);
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[ks], acc[rt], 0, 0, 0);
         }
     }
-    __syncthreads();  // every wave has read act
+    __syncthreads();  // every wave has read its input
     if (active) {
         const int col = col0 + l32;
         const float bc = bias[col];
@@ -98,39 +100,49 @@ __device__ void tile_layer(__bf16 *act, float *zt, const __bf16 *__restrict__ W,
 
 struct TrajShared {
     __bf16 act[TT * APITCH];
+    __bf16 xin[TT * XPITCH];   // network input; columns P.. and rows past C stay zero
     float zt[TT * (KOUT + 1)];
     double phi[TT * PP], cand[TT * PP], xi[TT * PP];
     double L[TRAJ_PMAX * TRAJ_PMAX];
     double lo[TRAJ_PMAX], hi[TRAJ_PMAX], pa[TRAJ_PMAX], pb[TRAJ_PMAX], pconst[TRAJ_PMAX];
     int kind[TRAJ_PMAX];
     double e[KOUT], s[KOUT];
-    double prior_phi[TT], post_phi[TT], cprior[TT];
+    double prior_phi[TT], post_phi[TT], cprior[TT], sllk[TT];
     int valid[TT];
 };
 
-// network input of the tile's rows from a chain tile (rows past C are zero)
+// network input of the tile's rows from a chain tile: the first P columns
+// of the rows below C (the rest of xin is zeroed once per launch)
 __device__ void write_input(TrajShared &sh, const double *rows, int c0, const TrajArgs &a)
 {
-    for (int it = threadIdx.x; it < TT * KIN; it += THREADS) {
-        const int r = it / KIN, j = it % KIN;
-        float v = 0.0f;
-        if (j < a.P && c0 + r < a.C) v = (float)(2.0 * (rows[r * PP + j] - sh.lo[j]) / (sh.hi[j] - sh.lo[j]) - 1.0);
-        sh.act[r * APITCH + j] = (__bf16)v;
+    for (int it = threadIdx.x; it < TT * a.P; it += THREADS) {
+        const int r = it / a.P, j = it % a.P;
+        if (c0 + r < a.C)
+            sh.xin[r * XPITCH + j] = (__bf16)(float)(2.0 * (rows[r * PP + j] - sh.lo[j]) / (sh.hi[j] - sh.lo[j]) - 1.0);
     }
 }
 
 __device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
 {
     __syncthreads();
-    tile_layer<KIN, HID, false>(sh.act, sh.zt, a.W[0], a.bias[0], lane, wid);
-    for (int l = 1; l < a.nlayers - 1; ++l) tile_layer<HID, HID, false>(sh.act, sh.zt, a.W[l], a.bias[l], lane, wid);
-    tile_layer<HID, KOUT, true>(sh.act, sh.zt, a.W[a.nlayers - 1], a.bias[a.nlayers - 1], lane, wid);
+    tile_layer<KIN, HID, false>(sh.xin, XPITCH, sh.act, sh.zt, a.W[0], a.bias[0], lane, wid);
+    for (int l = 1; l < a.nlayers - 1; ++l)
+        tile_layer<HID, HID, false>(sh.act, APITCH, sh.act, sh.zt, a.W[l], a.bias[l], lane, wid);
+    tile_layer<HID, KOUT, true>(sh.act, APITCH, sh.act, sh.zt, a.W[a.nlayers - 1], a.bias[a.nlayers - 1], lane, wid);
 }
 
-// surrogate_llk_kernel's sum for tile row r
-__device__ double surrogate_llk_row(const TrajShared &sh, int r, const TrajArgs &a)
+// surrogate_llk_kernel's sum for every tile row into sh.sllk: threads
+// 4 r .. 4 r + 3 take the four partial sums of row r, the first adds them in order
+__device__ void tile_llk(TrajShared &sh, const TrajArgs &a)
 {
-    return surrogate_llk_sum(sh.zt + r * (KOUT + 1), sh.e, sh.s, a.K, a.cperp, a.norm);
+    const int tid = threadIdx.x, lane = tid & 63;
+    double part = 0.0;
+    if (tid < SLLK_PARTS * TT)
+        part = surrogate_llk_part(sh.zt + (tid >> 2) * (KOUT + 1), sh.e, sh.s, a.K, tid & 3);
+    const int src = lane < 61 ? lane : 60;  // lanes 61-63 take no part in a combination
+    const double p1 = __shfl(part, src + 1, 64), p2 = __shfl(part, src + 2, 64), p3 = __shfl(part, src + 3, 64);
+    if (tid < SLLK_PARTS * TT && (tid & 3) == 0) sh.sllk[tid >> 2] = surrogate_llk_combine(part, p1, p2, p3, a.cperp, a.norm);
+    __syncthreads();
 }
 
 }  // namespace
@@ -160,17 +172,19 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         const int r = it / P, j = it % P;
         if (c0 + r < a.C) sh.phi[r * PP + j] = a.theta[(size_t)(c0 + r) * P + j];
     }
+    for (int it = tid; it < TT * XPITCH; it += THREADS) sh.xin[it] = (__bf16)0.0f;
     __syncthreads();
     // the surrogate posterior of the starting state
     write_input(sh, sh.phi, c0, a);
     network(sh, a, lane, wid);
+    tile_llk(sh, a);
     const int r = tid;  // the chain of a thread of wavefront 0
     const int c = c0 + r;
     const bool mine = tid < TT && c < a.C;
     int moved = 0;
     unsigned n_acc = 0, n_rej = 0, n_unl = 0;
     if (mine) {
-        const double llk = surrogate_llk_row(sh, r, a);
+        const double llk = sh.sllk[r];
         const double post = tempered(a.prior_theta[c], a.beta, llk);
         sh.prior_phi[r] = a.prior_theta[c];
         sh.post_phi[r] = post;
@@ -230,9 +244,10 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         __syncthreads();
         write_input(sh, sh.cand, c0, a);
         network(sh, a, lane, wid);
+        tile_llk(sh, a);
         // sub-step acceptance (da_traj_step_kernel)
         if (mine) {
-            const double cdata_s = surrogate_llk_row(sh, r, a);
+            const double cdata_s = sh.sllk[r];
             const double cp = tempered(sh.cprior[r], a.beta, cdata_s);
             if (!sh.valid[r]) {
                 ++n_rej;
