@@ -79,6 +79,7 @@ SIGNATURES = {
     "altar_valu_probe": (_i, [_i, _vp]),
     "altar_gemm_f32": (_i, [_vp, _sz, _vp, _sz, _i, _i, _i, _vp, _vp, _sz, _vp]),
     "altar_gather_rows_f32": (_i, [_vp, _sz, _vp, _i, _i, _vp, _sz, _vp]),
+    "altar_transpose_f32": (_i, [_vp, _sz, _i, _i, _vp, _sz, _vp]),
     "altar_mlp_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _i]),
     "altar_mlp_destroy": (_i, [_vp]),
     "altar_mlp_input_scaling": (_i, [_vp, _vp, _vp]),

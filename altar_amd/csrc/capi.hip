@@ -893,6 +893,17 @@ int altar_gemm_f32(const float *A_dev, size_t lda, const float *B_dev, size_t ld
     });
 }
 
+// dst[c, r] = src[r, c] (FP32, rows x cols)
+int altar_transpose_f32(const float *src_dev, size_t lds, int rows, int cols, float *dst_dev, size_t ldd,
+                        void *stream)
+{
+    return guarded([&] {
+        require(src_dev && dst_dev && rows > 0 && cols > 0 && lds >= (size_t)cols && ldd >= (size_t)rows,
+                "bad argument");
+        hip_check(launch_transpose_f32(src_dev, lds, rows, cols, dst_dev, ldd, S(stream)), "transpose");
+    });
+}
+
 // dst[r, :cols] = src[idx[r], :cols] for r < n (FP32 rows)
 int altar_gather_rows_f32(const float *src_dev, size_t lds, const int *idx_dev, int n, int cols, float *dst_dev,
                           size_t ldd, void *stream)

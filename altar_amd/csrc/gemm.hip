@@ -425,6 +425,34 @@ hipError_t launch_gather_rows_f32(const float *src, size_t lds, const int *idx, 
     return hipGetLastError();
 }
 
+// dst[c, r] = src[r, c] for r < rows, c < cols: 64 x 64 tiles through LDS
+// (pitch 65 floats, so the column reads hit distinct banks)
+__global__ __launch_bounds__(256) void transpose_f32_kernel(const float *__restrict__ src, size_t lds, int rows,
+                                                            int cols, float *__restrict__ dst, size_t ldd)
+{
+    __shared__ float tile[64][65];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4 threads
+    for (int j = ty; j < 64; j += 4) {
+        const int r = r0 + j, c = c0 + tx;
+        tile[j][tx] = (r < rows && c < cols) ? src[(size_t)r * lds + c] : 0.0f;
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        const int c = c0 + j, r = r0 + tx;
+        if (c < cols && r < rows) dst[(size_t)c * ldd + r] = tile[tx][j];
+    }
+}
+
+hipError_t launch_transpose_f32(const float *src, size_t lds, int rows, int cols, float *dst, size_t ldd,
+                                hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_f32_kernel, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(256), 0, st, src, lds, rows,
+                       cols, dst, ldd);
+    return hipGetLastError();
+}
+
 hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
                              double *llk, hipStream_t st)
 {

@@ -114,6 +114,8 @@ hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, cons
                               const GemmEpi &ep, hipStream_t st);
 hipError_t launch_gather_rows_f32(const float *src, size_t lds, const int *idx, int n, int cols, float *dst,
                                   size_t ldd, hipStream_t st);
+hipError_t launch_transpose_f32(const float *src, size_t lds, int rows, int cols, float *dst, size_t ldd,
+                                hipStream_t st);
 hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
                              double *llk, hipStream_t st);
 hipError_t launch_whiten_f64(const double *R, size_t ldr, const double *W, size_t ldw, int N, int C,

@@ -105,6 +105,60 @@ def whitened_pca(wr, K, oversample=32, iterations=3, seed=0):
     return mean, Vt[:K].T.copy(), sv[:K]
 
 
+def device_pca(store, ld, N, rows, K, oversample=32, iterations=3, seed=0):
+    """Mean and top-K right singular vectors of the field rows store[rows, :N]
+    (device FP32, row pitch ``ld``, a multiple of 128): the randomized SVD of
+    ``whitened_pca`` with the products against the field matrix X on the
+    device (FP32 MFMA GEMMs over X and its transpose) and the QR and SVD
+    factorizations of the thin factors in FP64 on the host.  ``len(rows)``
+    must be a multiple of 128 and K + oversample at most 128."""
+    dev = store.device
+    st = _lib.stream()
+    p = _lib.ptr
+    M = len(rows)
+    q = K + oversample
+    assert M % 128 == 0 and ld % 128 == 0 and q <= 128
+    X = torch.zeros((M, ld), dtype=torch.float32, device=dev)
+    idx = torch.from_numpy(np.asarray(rows, dtype=np.int32)).to(dev)
+    _lib.call("altar_gather_rows_f32", p(store), ld, p(idx), M, N, p(X), ld, st)
+    XT = torch.empty((ld, M), dtype=torch.float32, device=dev)
+    _lib.call("altar_transpose_f32", p(X), ld, M, ld, p(XT), M, st)
+    # column means: a block of ones times X (row 0 of the product)
+    ones = torch.ones((128, M), dtype=torch.float32, device=dev)
+    sums = torch.empty((128, ld), dtype=torch.float32, device=dev)
+    _lib.call("altar_gemm_f32", p(ones), M, p(XT), M, 128, ld, M, None, p(sums), ld, st)
+    mean = sums[0].double().cpu().numpy() / M
+    del ones, sums
+
+    def x_times(Bt):
+        """(X - 1 mean^T) Bt^T for Bt (q x ld): M x q"""
+        B = torch.zeros((128, ld), dtype=torch.float32, device=dev)
+        B[:q] = torch.from_numpy(Bt.astype(np.float32)).to(dev)
+        bias = torch.zeros(128, dtype=torch.float32, device=dev)
+        bias[:q] = torch.from_numpy((Bt @ mean).astype(np.float32)).to(dev)
+        out = torch.empty((M, 128), dtype=torch.float32, device=dev)
+        _lib.call("altar_gemm_f32", p(X), ld, p(B), ld, M, 128, ld, p(bias), p(out), 128, st)
+        return out[:, :q].double().cpu().numpy()
+
+    def xt_times(Q):
+        """(X - 1 mean^T)^T Q for Q (M x q): ld x q"""
+        B = torch.zeros((128, M), dtype=torch.float32, device=dev)
+        B[:q] = torch.from_numpy(np.ascontiguousarray(Q.T, dtype=np.float32)).to(dev)
+        out = torch.empty((ld, 128), dtype=torch.float32, device=dev)
+        _lib.call("altar_gemm_f32", p(XT), M, p(B), M, ld, 128, M, None, p(out), 128, st)
+        return out[:, :q].double().cpu().numpy() - np.outer(mean, Q.sum(axis=0))
+
+    rng = np.random.default_rng(seed)
+    omega = rng.standard_normal((q, ld))
+    omega[:, N:] = 0.0
+    Q = np.linalg.qr(x_times(omega))[0]
+    for _ in range(iterations):
+        Qn = np.linalg.qr(xt_times(Q))[0]
+        Q = np.linalg.qr(x_times(Qn.T))[0]
+    _, sv, Vt = np.linalg.svd(xt_times(Q).T, full_matrices=False)
+    return mean[:N].copy(), Vt[:K, :N].T.copy(), sv[:K]
+
+
 class Surrogate:
     """PCA + MLP surrogate of a source's whitened residual field."""
 
@@ -163,14 +217,10 @@ class Surrogate:
         perm = rng.permutation(n)
         nval = max(256, int(holdout * n))
         val_idx, tr_idx = perm[:nval], perm[nval:]
-        # PCA on a host sample of the training rows (whitened residuals)
-        samp = np.sort(rng.choice(tr_idx, size=min(pca_sample, len(tr_idx)), replace=False))
-        idx_dev = torch.from_numpy(samp.astype(np.int32)).to(dev)
-        tmp = torch.empty((len(samp), self.N), dtype=torch.float32, device=dev)
-        _lib.call("altar_gather_rows_f32", _lib.ptr(self.wr), self.Npad, _lib.ptr(idx_dev), len(samp), self.N,
-                  _lib.ptr(tmp), self.N, _lib.stream())
-        mean, V, sv = whitened_pca(tmp.cpu().numpy(), self.K, seed=self.seed)
-        del tmp
+        # PCA of a sample of the training rows (whitened residuals), products on the device
+        m = max(128, min(pca_sample, len(tr_idx)) // 128 * 128)
+        samp = np.sort(rng.choice(tr_idx, size=m, replace=len(tr_idx) < m))
+        mean, V, sv = device_pca(self.wr, self.Npad, self.N, samp, self.K, seed=self.seed)
         self.mean, self.V = mean, V
         self.e = V.T @ mean
         self.cperp = float(mean @ mean - self.e @ self.e)

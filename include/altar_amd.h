@@ -240,6 +240,9 @@ int altar_posterior(const double *prior_dev, const double *data_dev, double beta
  * M, N multiples of 128, K a multiple of 16 */
 int altar_gemm_f32(const float *A_dev, size_t lda, const float *B_dev, size_t ldb, int M, int N, int K,
                    const float *bias_dev, float *out_dev, size_t ldo, void *stream);
+/* dst[c, r] = src[r, c] for r < rows, c < cols (FP32, row pitches lds, ldd) */
+int altar_transpose_f32(const float *src_dev, size_t lds, int rows, int cols, float *dst_dev, size_t ldd,
+                        void *stream);
 /* dst[r, 0:cols] = src[idx[r], 0:cols], r < n */
 int altar_gather_rows_f32(const float *src_dev, size_t lds, const int *idx_dev, int n, int cols, float *dst_dev,
                           size_t ldd, void *stream);

@@ -196,3 +196,26 @@ def test_trajectory_kernels_match_restatement(cuda, oracle):
     n = int(count.item())
     assert np.array_equal(index[:n].cpu().numpy(), np.nonzero(moved)[0])
     assert counters.cpu().tolist() == [n_acc, n_rej, n_unl]
+
+
+def test_device_pca_matches_host_routine(cuda):
+    """device_pca (products on the device, FP32 MFMA) against whitened_pca
+    (host FP64) on rank-20 fields plus noise and a mean offset: the same
+    mean, singular values and subspace."""
+    from altar_amd.surrogate import device_pca, whitened_pca
+    print("case: device PCA 1024 rows x 700 columns (pitch 768), K=16", flush=True)
+    rng = np.random.default_rng(12)
+    M, N, ld, K = 1024, 700, 768, 16
+    U = rng.normal(size=(M, 20))
+    Vb = rng.normal(size=(20, N)) * np.linspace(3.0, 0.3, 20)[:, None]
+    F = U @ Vb + 0.01 * rng.normal(size=(M, N)) + rng.normal(size=N)
+    rows = np.sort(rng.choice(M + 256, M, replace=False))
+    store = torch.zeros((M + 256, ld), dtype=torch.float32, device=cuda)
+    store[torch.from_numpy(rows).to(cuda), :N] = torch.from_numpy(F.astype(np.float32)).to(cuda)
+    mean_d, V_d, sv_d = device_pca(store, ld, N, rows, K, seed=4)
+    mean_h, V_h, sv_h = whitened_pca(F.astype(np.float32), K, seed=4)
+    print("mean err", np.max(np.abs(mean_d - mean_h)), "sv rel err", np.max(np.abs(sv_d / sv_h - 1)))
+    assert np.max(np.abs(mean_d - mean_h)) < 1e-5 * np.max(np.abs(F))
+    assert np.allclose(sv_d, sv_h, rtol=1e-4)
+    cosines = np.linalg.svd(V_d.T @ V_h, compute_uv=False)
+    assert cosines.min() > 1 - 1e-4
