@@ -49,7 +49,7 @@ constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
 // half * 16 + kk * 8 for kk = 0, 1.
 template <int DIN, int DOUT, bool LAST>
 __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt, const __bf16 *__restrict__ W,
-                           const float *__restrict__ bias, int lane, int wid)
+                           int wpitch, const float *__restrict__ bias, int lane, int wid)
 {
     constexpr int RT = DOUT == HID ? 2 : 1;
     const int half = lane >> 5, l32 = lane & 31;
@@ -65,7 +65,7 @@ __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt,
     // (DIN / 16 16-byte loads, at most 64 registers), then the MFMA chain
     constexpr int KS = DIN / 16;
     if (active) {
-        const __bf16 *wrow = W + (size_t)(col0 + l32) * DIN + half * 16;
+        const __bf16 *wrow = W + (size_t)(col0 + l32) * wpitch + half * 16;
         bf16x8 b[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) b[ks] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
@@ -125,10 +125,13 @@ __device__ void write_input(TrajShared &sh, const double *rows, int c0, const Tr
 __device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
 {
     __syncthreads();
-    tile_layer<KIN, HID, false>(sh.xin, XPITCH, sh.act, sh.zt, a.W[0], a.bias[0], lane, wid);
+    // layer 0 over k < 32 only: the inputs past P <= 32 are zero, and zero
+    // products leave the MFMA sums unchanged (mlp_gemm_kernel runs all 64)
+    tile_layer<TRAJ_PMAX, HID, false>(sh.xin, XPITCH, sh.act, sh.zt, a.W[0], KIN, a.bias[0], lane, wid);
     for (int l = 1; l < a.nlayers - 1; ++l)
-        tile_layer<HID, HID, false>(sh.act, APITCH, sh.act, sh.zt, a.W[l], a.bias[l], lane, wid);
-    tile_layer<HID, KOUT, true>(sh.act, APITCH, sh.act, sh.zt, a.W[a.nlayers - 1], a.bias[a.nlayers - 1], lane, wid);
+        tile_layer<HID, HID, false>(sh.act, APITCH, sh.act, sh.zt, a.W[l], HID, a.bias[l], lane, wid);
+    tile_layer<HID, KOUT, true>(sh.act, APITCH, sh.act, sh.zt, a.W[a.nlayers - 1], HID, a.bias[a.nlayers - 1], lane,
+                                wid);
 }
 
 // surrogate_llk_kernel's sum for every tile row into sh.sllk: threads
