@@ -1,6 +1,6 @@
 """Learned surrogate of the forward model for delayed-acceptance screening.
 
-No reference counterpart: this is the north-star addition to CATMIP.  A
+No reference counterpart: an addition to CATMIP.  A
 dense network (10 -> 256 -> 256 -> 256 -> 64, GELU, BF16 weights, FP32
 accumulation; kernels in csrc/mlp.hip) maps source parameters to the leading
 principal components of the *whitened* residual field W r(theta), where W is

@@ -157,7 +157,7 @@ def main():
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(chain-parallel sharding is described in DESIGN.md, not built)"}))
+                          "(chain-parallel sharding is a design, not built)"}))
         return 2
     if args.workload != "cdm":
         print(json.dumps(side_workload(args)), flush=True)

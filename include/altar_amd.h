@@ -248,7 +248,7 @@ int altar_gather_rows_f32(const float *src_dev, size_t lds, const int *idx_dev, 
                           size_t ldd, void *stream);
 
 /* ---- learned surrogate (no reference counterpart: the delayed-acceptance
- * screen of the north-star design).  A dense network nin -> nhid x
+ * screen; an addition to CATMIP).  A dense network nin -> nhid x
  * (nlayers - 1) -> nout with GELU hidden activations, BF16 weights with FP32
  * master copies and Adam, FP32 accumulation on the matrix cores.  Inputs are
  * parameter vectors scaled to [-1, 1] by altar_mlp_input_scaling. */
