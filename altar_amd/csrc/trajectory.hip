@@ -107,7 +107,7 @@ struct TrajShared {
     double lo[TRAJ_PMAX], hi[TRAJ_PMAX], pa[TRAJ_PMAX], pb[TRAJ_PMAX], pconst[TRAJ_PMAX];
     int kind[TRAJ_PMAX];
     double e[KOUT], s[KOUT];
-    double prior_phi[TT], post_phi[TT], cprior[TT], sllk[TT];
+    double prior_phi[TT], post_phi[TT], cprior[TT], sllk[TT], logu[TT];
     int valid[TT];
 };
 
@@ -196,8 +196,17 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
     const int pairs = (P + 1) / 2;
     for (int sub = 0; sub < a.substeps; ++sub) {
         const uint32_t step2 = a.step2 + (uint32_t)sub;
-        // normals (normals_kernel)
-        for (int it = tid; it < TT * pairs; it += THREADS) {
+        // normals (normals_kernel), and the log of each chain's sub-step dice
+        // (da_traj_step_kernel's), which do not depend on the sub-step's outcome
+        for (int it = tid; it < TT * (pairs + 1); it += THREADS) {
+            if (it >= TT * pairs) {
+                const int rr = it - TT * pairs;
+                if (c0 + rr < a.C) {
+                    u32x4 rn = draw(a.seed, a.stage, step2, (uint32_t)(c0 + rr), TAG_TRAJ_ACCEPT, 0u);
+                    sh.logu[rr] = log(u53(rn.x, rn.y));
+                }
+                continue;
+            }
             const int rr = it / pairs, w = it % pairs;
             if (c0 + rr >= a.C) continue;
             u32x4 rn = draw(a.seed, a.stage, step2, (uint32_t)(c0 + rr), TAG_TRAJ_PROPOSAL, (uint32_t)w);
@@ -255,9 +264,7 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
             if (!sh.valid[r]) {
                 ++n_rej;
             } else {
-                u32x4 rn = draw(a.seed, a.stage, step2, (uint32_t)c, TAG_TRAJ_ACCEPT, 0u);
-                const double u = u53(rn.x, rn.y);
-                if (log(u) <= cp - sh.post_phi[r]) {
+                if (sh.logu[r] <= cp - sh.post_phi[r]) {
                     ++n_acc;
                     moved = 1;
                     for (int j = 0; j < P; ++j) sh.phi[r * PP + j] = sh.cand[r * PP + j];
