@@ -75,6 +75,9 @@ def parse():
     ap.add_argument("--joint-da", action="store_true",
                     help="joint workload: also train the surrogate and time delayed acceptance against plain steps")
     ap.add_argument("--da-train", type=int, default=200_000, help="training fields for the surrogate")
+    ap.add_argument("--da-stage-steps", type=int, default=13,
+                    help="Metropolis steps per annealing stage before the timed DA stage (training fields are "
+                         "collected while beta < 0.1: 13 steps of 16,384 chains fill the 200,000-field store)")
     ap.add_argument("--da-substeps", default="128",
                     help="surrogate sub-steps per exact step in the trajectory leg, or a comma-separated "
                          "list (the first is the reported one; 1 disables the leg)")
@@ -262,7 +265,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(case, theta, args.cpu_seconds)
         out["cpu_baseline_all_cores"] = cpu_baseline(case, theta, args.cpu_seconds / 2, omp=True)
     if not args.no_da:
-        out["surrogate_da"] = da_leg(args, model, C)
+        out["surrogate_da"] = da_leg(args, model, C, stage_steps=args.da_stage_steps)
     print(json.dumps(out), flush=True)
     return 0
 
