@@ -333,6 +333,93 @@ class Surrogate:
             count += f.size
         return float(np.sqrt(sq / count))
 
+    # -- field RMSE on the device (the host forms above are the reference)
+    def _rmse_blocks(self):
+        """(offset, n, map) per dataset on the device, where map takes a
+        whitened-residual difference back to the LOS field: chol(Cd) (padded
+        to a multiple of 128, FP32) for a full Cd in exact mode, W^-1 in the
+        other modes, the standard deviations for a diagonal Cd."""
+        blocks = getattr(self, "_Lb_dev", None)
+        if blocks is not None:
+            return blocks
+        from . import norms
+        dev = self.theta.device
+        if getattr(self.model, "block_offsets", None) is not None:
+            parts = []
+            for off, ds in zip(self.model.block_offsets, self.model.datasets):
+                cd = np.asarray(ds.cd, dtype=np.float64)
+                if cd.ndim == 1:
+                    parts.append((int(off), len(ds.x), np.sqrt(cd)))
+                else:
+                    L = ds.chol if ds.chol is not None else np.linalg.cholesky(cd)
+                    if getattr(self.model, "whiten", "exact") != "exact":
+                        L = np.linalg.inv(norms.whitener(cd, self.model.whiten)[1])
+                    parts.append((int(off), len(ds.x), L))
+        elif self.model.whitening_kind == "lower":
+            if getattr(self.model, "whiten", "exact") == "exact":
+                L = np.linalg.cholesky(np.asarray(self.model.cd, dtype=np.float64))
+            else:
+                L = np.linalg.inv(self.model.W)
+            parts = [(0, self.N, L)]
+        else:
+            parts = [(0, self.N, 1.0 / np.asarray(self.model.W, dtype=np.float64))]
+        blocks = []
+        for off, n, L in parts:
+            if L.ndim == 1:
+                blocks.append((off, n, torch.from_numpy(L.astype(np.float32)).to(dev)))
+            else:
+                npad = (n + 127) // 128 * 128
+                Lp = np.zeros((npad, npad), dtype=np.float32)
+                Lp[:n, :n] = L
+                blocks.append((off, n, torch.from_numpy(Lp).to(dev)))
+        self._Lb_dev = blocks
+        return blocks
+
+    def _v_device(self):
+        """The basis V (Npad x K, FP32, zero rows past N) and -mean on the device."""
+        cached = getattr(self, "_Vdev", None)
+        if cached is not None and cached[0] is self.V:
+            return cached[1], cached[2]
+        dev = self.theta.device
+        Vp = np.zeros((self.Npad, self.K), dtype=np.float32)
+        Vp[:self.N] = self.V
+        negmean = np.zeros(self.Npad, dtype=np.float32)
+        negmean[:self.N] = -self.mean
+        self._Vdev = (self.V, torch.from_numpy(Vp).to(dev), torch.from_numpy(negmean).to(dev))
+        return self._Vdev[1], self._Vdev[2]
+
+    def field_rmse_device(self, theta, wr_true):
+        """RMSE in metres of the predicted LOS field against the exact one:
+        predicted whitened residuals mean + V (s * z) and the mapping back
+        through W^-1 are FP32 GEMMs on the device.  theta: rows x P (FP64),
+        wr_true: rows x (>= N) FP32 whitened residuals, both on the device."""
+        rows = theta.shape[0]
+        rp = (rows + 127) // 128 * 128
+        dev = theta.device
+        p, st = _lib.ptr, _lib.stream()
+        V, negmean = self._v_device()
+        z = self.mlp.forward(theta.contiguous())
+        A = torch.zeros((rp, self.K), dtype=torch.float32, device=dev)
+        A[:rows] = z[:, :self.K] * self.s_d.float()
+        pred = torch.empty((rp, self.Npad), dtype=torch.float32, device=dev)
+        _lib.call("altar_gemm_f32", p(A), self.K, p(V), self.K, rp, self.Npad, self.K, p(negmean), p(pred),
+                  self.Npad, st)
+        diff = pred[:rows, :self.N] - wr_true[:rows, :self.N]
+        sq, count = 0.0, 0
+        for off, n, L in self._rmse_blocks():
+            d = diff[:, off:off + n]
+            if L.dim() == 1:
+                sq += float(((d * L) ** 2).sum())
+            else:
+                npad = L.shape[0]
+                Ad = torch.zeros((rp, npad), dtype=torch.float32, device=dev)
+                Ad[:rows, :n] = d
+                f = torch.empty((rp, npad), dtype=torch.float32, device=dev)
+                _lib.call("altar_gemm_f32", p(Ad), npad, p(L), npad, rp, npad, npad, None, p(f), npad, st)
+                sq += float((f[:rows, :n].double() ** 2).sum())
+            count += rows * n
+        return float(np.sqrt(sq / count))
+
     def validate_stored(self, idx):
         idx = np.sort(np.asarray(idx))[:4096]
         dev = self.theta.device
@@ -343,4 +430,4 @@ class Surrogate:
         wr = torch.empty((len(idx), self.N), dtype=torch.float32, device=dev)
         _lib.call("altar_gather_rows_f32", _lib.ptr(self.wr), self.Npad, _lib.ptr(idx_d), len(idx), self.N,
                   _lib.ptr(wr), self.N, _lib.stream())
-        return self.field_rmse(self.predict_wr(th), wr.double().cpu().numpy())
+        return self.field_rmse_device(th, wr)

@@ -151,3 +151,16 @@ def test_joint_fields_and_surrogate(cuda, oracle):
             break
     print("joint DA modes", used, "validation rmse", sur.validation_rmse)
     assert sur.trained and "da" in used
+    # the device field RMSE (GEMMs through each dataset's chol(Cd_b)) against
+    # the host form on the current chains
+    st = w.step
+    th = st.theta[:512].contiguous()
+    Np2 = sur.Npad
+    wr2 = torch.empty((512, Np2), dtype=torch.float32, device=cuda)
+    llk2 = torch.empty(512, dtype=torch.float64, device=cuda)
+    _lib.call("altar_source_data_llk_fields", model.source.handle, _lib.ptr(th), 512, _lib.ptr(llk2), _lib.FP32,
+              _lib.ptr(wr2), Np2, None)
+    dev_rmse = sur.field_rmse_device(th, wr2)
+    host_rmse = sur.field_rmse(sur.predict_wr(th), wr2[:, :sur.N].double().cpu().numpy())
+    print("field rmse device", dev_rmse, "host", host_rmse)
+    assert abs(dev_rmse / host_rmse - 1) < 1e-3
