@@ -440,7 +440,7 @@ def fused_kernel_timing(ann, da, start, subs, reps=3):
             "network_flops_per_launch": flops, "substeps": subs}
 
 
-def moment_check(ann, da, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equilibrate=100):
+def moment_check(ann, da, plain, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equilibrate=100):
     """Does the exact stage correct the surrogate's error?  Posterior means
     at one temperature by trajectory delayed acceptance with the trained
     surrogate and with a deliberately biased one (its PCA-coefficient offsets
@@ -456,7 +456,11 @@ def moment_check(ann, da, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equi
       its standard error: ``z_naive`` with sd / sqrt(reps C) per sampler
       (the chains have forgotten the start), ``z_rep`` with the replicate
       spread.  ``chi2`` = sum_k z_naive^2 (10 parameters: 18.3 is the 95%
-      point for independent z), shown for the control beside the test."""
+      point for independent z), shown for the control beside the test.
+    * Plain Metropolis from the same start (``steps`` steps, once): it moves
+      little, so its mean keeps the start's sampling error, sd / sqrt(C); the
+      trajectory runs have forgotten the start.  ``trained_vs_plain`` uses
+      that standard error for plain."""
     import torch
     sur = da.surrogate
     C = start.theta.shape[0]
@@ -484,6 +488,15 @@ def moment_check(ann, da, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equi
                 means.append(st.theta.mean(0))
                 sds.append(st.theta.std(0))
             res[name] = (torch.stack(means), torch.stack(sds).mean(0), acc / (reps * steps * C))
+        sur.e_d = e_true
+        ann.seed = seed0 + 7919 * (1 + 3 * reps)
+        ann.sampler = plain
+        keep_p = plain.steps
+        plain.steps = steps
+        st = base.clone()
+        a, _, _ = plain.walkChains(ann, st)
+        plain.steps = keep_p
+        res["plain"] = (st.theta.mean(0)[None, :], st.theta.std(0), a / (steps * C))
     finally:
         sur.e_d = e_true
         ann.seed = seed0
@@ -497,9 +510,13 @@ def moment_check(ann, da, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equi
         zr = d / torch.sqrt(Ma.var(0) / reps + Mb.var(0) / reps)
         return {"z_naive": zn.tolist(), "z_rep": zr.tolist(), "chi2": float((zn ** 2).sum()),
                 "max_abs_z": float(zn.abs().max()), "sd_ratio_max_deviation": float((sa / sb - 1).abs().max())}
+    (Mt, st_, _), (Mp, sp, _) = res["trained"], res["plain"]
+    zp = (Mt.mean(0) - Mp[0]) / torch.sqrt(st_ ** 2 / (reps * C) + sp ** 2 / C)
     return {"equilibrate_steps": equilibrate, "steps": steps, "replicates": reps, "bias_sd": bias_sd,
             "exact_acceptance": {k: v[2] for k, v in res.items()},
-            "trained_vs_biased": compare("trained", "biased"), "trained_vs_control": compare("trained", "control")}
+            "trained_vs_biased": compare("trained", "biased"), "trained_vs_control": compare("trained", "control"),
+            "trained_vs_plain": {"z": zp.tolist(), "chi2": float((zp ** 2).sum()), "max_abs_z": float(zp.abs().max()),
+                                 "sd_ratio_max_deviation": float((st_ / sp - 1).abs().max())}}
 
 
 def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwargs=None, validate_chains=1024,
@@ -623,7 +640,7 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
             subs = sweep[0]
             res["trajectory"]["fused_kernel"] = fused_kernel_timing(ann, da, start, subs)
             if moments and args.da_moment_steps > 0:
-                res["moments"] = moment_check(ann, da, start, args.da_moment_steps, subs)
+                res["moments"] = moment_check(ann, da, plain, start, args.da_moment_steps, subs)
         if len(runs) > 1:
             res["trajectory_sweep"] = runs
     ann.sampler = da
