@@ -268,3 +268,44 @@ def test_fused_trajectory_matches_separate_calls(cuda):
     for name in ("phi", "prior_phi", "post_phi", "post_s"):
         assert torch.equal(w0[name], w1[name]), name
     assert torch.equal(w0["index"][C:], w1["index"][C:])
+
+
+def test_substep_scale_tuning(cuda):
+    """Pilot trajectories tune the sub-step scale toward the target sub-step
+    acceptance and leave the chains where they were."""
+    print("case: sub-step scale tuning, linear-Gaussian C=4096 P=3", flush=True)
+    G, d, cd, S, mu = linear_problem()
+    P, C = 3, 4096
+    job = Job(chains=C, steps=5, seed=6, tolerance=1e-3)
+    model = Linear(default_psets(P), G, d, cd, mode="direct", job=job).initialize()
+    sur = Surrogate(model, K=32, capacity=C * 10, max_batch=C, seed=1)
+    da = DelayedAcceptance(sur, train_beta=0.05, max_rmse=1e9, steps=5, substeps=8,
+                           fit_kwargs=dict(epochs=20, batch=2048, lr=2e-3))
+    ann = Annealer(da, COV(), seed=6).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    while not sur.trained:
+        w.cool(ann)
+        w.resample(ann, w.walk(ann))
+    w.cool(ann)
+    step = w.step
+    da.prepareSamplingPDF(ann, step)
+    before = step.theta.clone()
+    rates = {}
+    for target in (0.2, 0.5):
+        da.substep_scale, da.substep_target = 4.0, target
+        scale = da.tune_substep_scale(ann, step, da._workspace(C, P, step.theta.device), rounds=8)
+        # the rate at the tuned scale, from the counters of one more pilot
+        cnt0 = list(da.substep_counts)
+        da.substep_target = None
+        da.steps = 1
+        st = step.clone()
+        da._walk_trajectory(ann, st)
+        c = [b - a for a, b in zip(cnt0, da.substep_counts)]
+        rates[target] = (scale, c[0] / max(sum(c), 1))
+    print("target -> (scale, sub-step acceptance):", rates)
+    assert torch.equal(step.theta, before)
+    assert rates[0.2][0] > rates[0.5][0]
+    for target, (_, rate) in rates.items():
+        assert abs(rate - target) < 0.1
