@@ -89,6 +89,9 @@ def parse():
                     help="sub-step acceptance rate the pilot trajectories tune the sub-step scale to (0: fixed)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
+    ap.add_argument("--trajectory-kernel", type=int, default=0, metavar="SUBSTEPS",
+                    help="only time altar_da_trajectory_fused on the CDM chains (SUBSTEPS sub-steps, network "
+                         "with its initial weights), for counter runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -121,6 +124,62 @@ def valu_calibration():
         _lib.call("altar_valu_probe", k, ctypes.byref(r))
         out[name] = {"lane_ops_per_s": r.value, "per_clk_per_simd": r.value / (1024 * 2.4e9)}
     print(json.dumps({"valu_calibration": out}))
+
+
+def trajectory_kernel(args, subs, reps=3):
+    """The fused trajectory kernel alone: CDM chains at the first annealing
+    temperature, the 10 -> 256 x 3 -> 64 network with its initial weights
+    and a fixed random PCA scaling (the kernel's cost does not depend on the
+    weights).  HIP-event time per launch on the launch stream."""
+    import torch
+    from altar_amd import Job, _lib
+    from altar_amd.bayesian import Annealer, COV, Metropolis
+    from altar_amd.cases import cdm_case
+    from altar_amd.cdm import CDM
+    from altar_amd.surrogate import Surrogate
+    torch.cuda.init()
+    C = args.chains
+    case = cdm_case(N=args.obs)
+    job = Job(chains=C, steps=1, seed=5)
+    model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"], nu=0.25,
+                job=job).initialize()
+    ann = Annealer(Metropolis(), COV(), seed=5).initialize(job)
+    ann.model = model
+    ann.worker.start(ann)
+    ann.worker.cool(ann)
+    step = ann.worker.step
+    ann.sampler.prepareSamplingPDF(ann, step)
+    sur = Surrogate(model, K=64, capacity=1024, max_batch=C, seed=2)
+    dev = step.theta.device
+    rng = np.random.default_rng(3)
+    e = torch.from_numpy(rng.normal(0.0, 1.0, 64)).to(dev)
+    s_ = torch.from_numpy(rng.uniform(0.05, 0.2, 64)).to(dev)
+    P = step.parameters
+    f64 = lambda *sh: torch.empty(sh, dtype=torch.float64, device=dev)
+    post_s, phi, pphi, postphi = f64(C), f64(C, P), f64(C), f64(C)
+    index = torch.empty(2 * C, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+    L = (ann.sampler.sigma_chol * 1.6).contiguous()
+    p = _lib.ptr
+
+    def launch(k):
+        _lib.call("altar_da_trajectory_fused", sur.mlp.handle, model._prior.handle, model.source.handle,
+                  p(step.theta), p(step.prior), p(post_s), p(phi), p(pphi), p(postphi), p(L), p(e), p(s_), 64, 3.0,
+                  float(model.normalization), float(step.beta), C, P, 5, 1, k, subs, p(index), p(cnt),
+                  _lib.stream())
+    launch(0)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for k in range(reps):
+        launch(k + 1)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    flops = 2.0 * (P * 256 + 2 * 256 * 256 + 256 * 64) * C * (subs + 1)
+    print(json.dumps({"trajectory_kernel": {"chains": C, "substeps": subs, "ms_per_launch": ms,
+                                            "us_per_substep": 1e3 * ms / subs,
+                                            "network_tflops": flops / (ms * 1e-3) / 1e12,
+                                            "substep_counts": cnt.tolist()}}))
 
 
 def cpu_baseline(case, theta, seconds, omp=False):
@@ -157,6 +216,9 @@ def main():
         return 0
     if args.valu_calibration:
         valu_calibration()
+        return 0
+    if args.trajectory_kernel:
+        trajectory_kernel(args, args.trajectory_kernel)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
