@@ -630,6 +630,7 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     ok = da._validate(ann, step)
     start = step.clone()
     sd = start.theta.std(0)
+    P = start.theta.shape[1]
     K = max(2, args.steps)
     da.steps = K
     plain = Metropolis(steps=K)
@@ -677,9 +678,17 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
         finally:
             ann.dispatcher.monitors.remove(mon)
         same = bool(torch.equal(st2.theta, st.theta))
+        esjd = mon.total / max(mon.steps, 1)
+        # effective samples: at stationarity ESJD = 2 sum_k (1 - rho_k) for
+        # standardized parameters, so rho = 1 - ESJD / (2 P) is the mean lag-1
+        # autocorrelation, and an AR(1) chain yields (1 - rho) / (1 + rho)
+        # effective samples per step
+        rho = min(max(1.0 - esjd / (2.0 * P), 0.0), 1.0)
+        ess_step = C * (1.0 - rho) / (1.0 + rho)
         return {"ms_per_step": dt / K * 1e3, "accepted": acc, "exact_evals": exact,
                 "accepted_samples_per_s": acc / dt, "exact_evals_per_s": exact / dt,
-                "esjd_per_step": mon.total / max(mon.steps, 1), "esjd_per_s": mon.total / dt,
+                "esjd_per_step": esjd, "esjd_per_s": mon.total / dt, "lag1_autocorrelation": rho,
+                "effective_samples_per_s": ess_step * K / dt,
                 "replay_identical": same, "final_means": st.theta.mean(0).tolist(),
                 "final_sds": st.theta.std(0).tolist()}
 
@@ -719,10 +728,13 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
         if r is not None and pl["accepted"]:
             r["gain_accepted_per_s"] = r["accepted_samples_per_s"] / pl["accepted_samples_per_s"]
             r["gain_esjd_per_s"] = r["esjd_per_s"] / pl["esjd_per_s"] if pl["esjd_per_s"] else None
+            r["gain_effective_samples_per_s"] = (r["effective_samples_per_s"] / pl["effective_samples_per_s"]
+                                                 if pl["effective_samples_per_s"] else None)
     for kind in ("da", "trajectory"):
         if kind in res and "gain_accepted_per_s" in res[kind]:
             out[f"{kind}_gain_accepted_per_s"] = res[kind]["gain_accepted_per_s"]
             out[f"{kind}_gain_esjd_per_s"] = res[kind]["gain_esjd_per_s"]
+            out[f"{kind}_gain_effective_samples_per_s"] = res[kind]["gain_effective_samples_per_s"]
     out["gain_accepted_per_s"] = out.get("da_gain_accepted_per_s")
     return out
 
