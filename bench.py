@@ -78,6 +78,7 @@ def parse():
     ap.add_argument("--da-stage-steps", type=int, default=13,
                     help="Metropolis steps per annealing stage before the timed DA stage (training fields are "
                          "collected while beta < 0.1: 13 steps of 16,384 chains fill the 200,000-field store)")
+    ap.add_argument("--da-epochs", type=int, default=100, help="training epochs of the surrogate network")
     ap.add_argument("--da-substeps", default="128",
                     help="surrogate sub-steps per exact step in the trajectory leg, or a comma-separated "
                          "list (the first is the reported one; 1 disables the leg)")
@@ -601,7 +602,7 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     t0 = time.perf_counter()
     sur = Surrogate(model, K=64, capacity=capacity or args.da_train, max_batch=C, seed=3)
     da = DelayedAcceptance(sur, train_beta=0.1, max_rmse=2e-3, steps=stage_steps, validate_chains=validate_chains,
-                           fit_kwargs=fit_kwargs or dict(epochs=40, batch=4096, lr=2e-3))
+                           fit_kwargs=fit_kwargs or dict(epochs=args.da_epochs, batch=4096, lr=2e-3))
     ann = Annealer(da, COV(), seed=job.seed).initialize(job)
     ann.model = model
     w = ann.worker
