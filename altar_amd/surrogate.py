@@ -392,7 +392,18 @@ class Surrogate:
         """RMSE in metres of the predicted LOS field against the exact one:
         predicted whitened residuals mean + V (s * z) and the mapping back
         through W^-1 are FP32 GEMMs on the device.  theta: rows x P (FP64),
-        wr_true: rows x (>= N) FP32 whitened residuals, both on the device."""
+        wr_true: rows x (>= N) FP32 whitened residuals, both on the device;
+        taken in chunks of the network's batch size."""
+        sq, count = 0.0, 0
+        step = self.mlp.max_batch
+        for r0 in range(0, theta.shape[0], step):
+            s_, c_ = self._field_sq(theta[r0:r0 + step], wr_true[r0:r0 + step])
+            sq += s_
+            count += c_
+        return float(np.sqrt(sq / count))
+
+    def _field_sq(self, theta, wr_true):
+        """Sum of squared LOS-field errors and their count for one chunk."""
         rows = theta.shape[0]
         rp = (rows + 127) // 128 * 128
         dev = theta.device
@@ -418,7 +429,7 @@ class Surrogate:
                 _lib.call("altar_gemm_f32", p(Ad), npad, p(L), npad, rp, npad, npad, None, p(f), npad, st)
                 sq += float((f[:rows, :n].double() ** 2).sum())
             count += rows * n
-        return float(np.sqrt(sq / count))
+        return sq, count
 
     def validate_stored(self, idx):
         idx = np.sort(np.asarray(idx))[:4096]
