@@ -88,6 +88,8 @@ SIGNATURES = {
     "altar_mlp_forward": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "altar_mlp_backward": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp, _vp]),
     "altar_mlp_adam": (_i, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]),
+    "altar_mlp_train_epoch": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _i, _i, _i, _i, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     "altar_surrogate_llk": (_i, [_vp, _i, _i, _i, _vp, _vp, _d, _d, _vp, _vp]),
     "altar_da_screen": (_i, [_vp, _vp, _vp, _vp, _d, _i, _u64, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
     "altar_da_pack": (_i, [_vp, _vp, _i, _i, _vp, _vp]),

@@ -1233,6 +1233,7 @@ struct altar_mlp {
     std::vector<DevBuf> pre, act16, actT16, dY16, dYT16;
     DevBuf x16, xT16, out32, lo, hi, loss, tgt, wk;
     DevBuf gpart, cpart;                    // split-k and column-sum partials of the backward pass
+    DevBuf xb, yb;                          // a training batch (altar_mlp_train_epoch)
     long step = 0;
     int P = 0;
 };
@@ -1302,6 +1303,42 @@ void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int 
     }
 }
 
+// gradients of the weighted squared error over a batch of B rows (no update)
+void mlp_batch_gradients(altar_mlp *h, const double *theta_dev, const float *target_dev, int ldt, const float *w_dev,
+                         int B, float *loss_dev, hipStream_t st)
+{
+    const int Bp = (int)round_up((size_t)B, 64);
+    // rows B..Bp-1 of the input are zeroed, their targets are ignored
+    hip_check(hipMemsetAsync(h->x16.p, 0, (size_t)h->maxB * h->kin * 2, st), "memset");
+    mlp_forward(h, theta_dev, B, st);
+    // targets and weights into the padded (maxB x kout) layout
+    hip_check(hipMemsetAsync(h->tgt.p, 0, (size_t)h->maxB * h->kout * 4, st), "memset");
+    hip_check(hipMemsetAsync(h->wk.p, 0, (size_t)h->kout * 4, st), "memset");
+    hip_check(hipMemcpy2DAsync(h->tgt.p, h->kout * 4, target_dev, ldt * 4, h->nout * 4, B,
+                               hipMemcpyDeviceToDevice, st),
+              "targets");
+    hip_check(hipMemcpyAsync(h->wk.p, w_dev, h->nout * 4, hipMemcpyDeviceToDevice, st), "weights");
+    hip_check(hipMemsetAsync(h->loss.p, 0, 4, st), "memset");
+    mlp_backward(h, h->tgt.as<float>(), h->wk.as<float>(), Bp, B, st);
+    if (loss_dev) hip_check(hipMemcpyAsync(loss_dev, h->loss.p, 4, hipMemcpyDeviceToDevice, st), "loss");
+}
+
+void mlp_adam_step(altar_mlp *h, float lr, float beta1, float beta2, float eps, hipStream_t st)
+{
+    h->step += 1;
+    const float c1 = 1.0f - std::pow(beta1, (float)h->step), c2 = 1.0f - std::pow(beta2, (float)h->step);
+    for (int l = 0; l < h->nlayers; ++l) {
+        const int O = h->dout[l], K = h->din[l];
+        hip_check(launch_adam(h->W32[l].as<float>(), h->Wm[l].as<float>(), h->Wv[l].as<float>(),
+                              h->gW[l].as<float>(), O, K, lr, beta1, beta2, eps, c1, c2, h->W16[l].p,
+                              h->WT16[l].p, st),
+                  "adam W");
+        hip_check(launch_adam(h->b32[l].as<float>(), h->bm[l].as<float>(), h->bv[l].as<float>(),
+                              h->gb[l].as<float>(), 1, O, lr, beta1, beta2, eps, c1, c2, nullptr, nullptr, st),
+                  "adam b");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1363,7 +1400,7 @@ int altar_mlp_destroy(altar_mlp *h)
                         &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16})
             for (auto &b : *v) b.release();
         for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk, &h->gpart,
-                          &h->cpart})
+                          &h->cpart, &h->xb, &h->yb})
             b->release();
         delete h;
     });
@@ -1449,21 +1486,7 @@ int altar_mlp_backward(altar_mlp *h, const double *theta_dev, const float *targe
 {
     return guarded([&] {
         require(h && theta_dev && target_dev && w_dev && B > 0 && B <= h->maxB, "bad argument");
-        hipStream_t st = S(stream);
-        const int Bp = (int)round_up((size_t)B, 64);
-        // rows B..Bp-1 of the input are zeroed, their targets are ignored
-        hip_check(hipMemsetAsync(h->x16.p, 0, (size_t)h->maxB * h->kin * 2, st), "memset");
-        mlp_forward(h, theta_dev, B, st);
-        // targets and weights into the padded (maxB x kout) layout
-        hip_check(hipMemsetAsync(h->tgt.p, 0, (size_t)h->maxB * h->kout * 4, st), "memset");
-        hip_check(hipMemsetAsync(h->wk.p, 0, (size_t)h->kout * 4, st), "memset");
-        hip_check(hipMemcpy2DAsync(h->tgt.p, h->kout * 4, target_dev, ldt * 4, h->nout * 4, B,
-                                   hipMemcpyDeviceToDevice, st),
-                  "targets");
-        hip_check(hipMemcpyAsync(h->wk.p, w_dev, h->nout * 4, hipMemcpyDeviceToDevice, st), "weights");
-        hip_check(hipMemsetAsync(h->loss.p, 0, 4, st), "memset");
-        mlp_backward(h, h->tgt.as<float>(), h->wk.as<float>(), Bp, B, st);
-        if (loss_dev) hip_check(hipMemcpyAsync(loss_dev, h->loss.p, 4, hipMemcpyDeviceToDevice, st), "loss");
+        mlp_batch_gradients(h, theta_dev, target_dev, ldt, w_dev, B, loss_dev, S(stream));
     });
 }
 
@@ -1472,18 +1495,37 @@ int altar_mlp_adam(altar_mlp *h, float lr, float beta1, float beta2, float eps, 
 {
     return guarded([&] {
         require(h && lr > 0, "bad argument");
+        mlp_adam_step(h, lr, beta1, beta2, eps, S(stream));
+    });
+}
+
+// One epoch of training in one call: batch b takes the rows order[b * batch ..
+// (b + 1) * batch) of theta_store (FP64, P columns) and of targets (FP32, row
+// pitch ldt); each batch runs the gradients of altar_mlp_backward and one
+// Adam step with the cosine-decayed rate
+// lr_final + (lr - lr_final) (1 + cos(pi it / (total - 1))) / 2, it = it0 + b.
+// loss_dev (optional) receives the last batch's loss.
+int altar_mlp_train_epoch(altar_mlp *h, const double *theta_store_dev, const float *targets_dev, int ldt,
+                          const float *w_dev, const int *order_dev, int nbatches, int batch, int it0, int total,
+                          float lr, float lr_final, float beta1, float beta2, float eps, float *loss_dev, void *stream)
+{
+    return guarded([&] {
+        require(h && theta_store_dev && targets_dev && w_dev && order_dev && nbatches > 0 && batch > 0 &&
+                    batch <= h->maxB && ldt >= h->nout && total > 0 && lr > 0,
+                "bad argument");
         hipStream_t st = S(stream);
-        h->step += 1;
-        const float c1 = 1.0f - std::pow(beta1, (float)h->step), c2 = 1.0f - std::pow(beta2, (float)h->step);
-        for (int l = 0; l < h->nlayers; ++l) {
-            const int O = h->dout[l], K = h->din[l];
-            hip_check(launch_adam(h->W32[l].as<float>(), h->Wm[l].as<float>(), h->Wv[l].as<float>(),
-                                  h->gW[l].as<float>(), O, K, lr, beta1, beta2, eps, c1, c2, h->W16[l].p,
-                                  h->WT16[l].p, st),
-                      "adam W");
-            hip_check(launch_adam(h->b32[l].as<float>(), h->bm[l].as<float>(), h->bv[l].as<float>(),
-                                  h->gb[l].as<float>(), 1, O, lr, beta1, beta2, eps, c1, c2, nullptr, nullptr, st),
-                      "adam b");
+        h->xb.reserve((size_t)h->maxB * h->P * 8);
+        h->yb.reserve((size_t)h->maxB * h->nout * 4);
+        for (int b = 0; b < nbatches; ++b) {
+            const int *sel = order_dev + (size_t)b * batch;
+            hip_check(launch_da_pack(theta_store_dev, sel, batch, h->P, h->xb.as<double>(), st), "batch rows");
+            hip_check(launch_gather_rows_f32(targets_dev, ldt, sel, batch, h->nout, h->yb.as<float>(), h->nout, st),
+                      "batch targets");
+            mlp_batch_gradients(h, h->xb.as<double>(), h->yb.as<float>(), h->nout, w_dev, batch,
+                                b == nbatches - 1 ? loss_dev : nullptr, st);
+            const double frac = (double)(it0 + b) / (double)std::max(1, total - 1);
+            const double cur = lr_final + 0.5 * ((double)lr - lr_final) * (1.0 + std::cos(M_PI * frac));
+            mlp_adam_step(h, (float)cur, beta1, beta2, eps, st);
         }
     });
 }

@@ -246,32 +246,19 @@ class Surrogate:
         w_d = torch.from_numpy(w.astype(np.float32)).to(dev)
         self.e_d = torch.from_numpy(self.e).to(dev)
         self.s_d = torch.from_numpy(s).to(dev)
-        # training loop over shuffled mini-batches (gathered rows)
+        # training: one native call per epoch over shuffled mini-batches
+        # (gathered rows, backward pass, Adam with a cosine-decayed rate)
         tr = np.asarray(tr_idx)
         batch = min(batch, self.max_batch)
         nb = max(1, len(tr) // batch)
-        xb = torch.empty((batch, self.P), dtype=torch.float64, device=dev)
-        yb = torch.empty((batch, self.K), dtype=torch.float32, device=dev)
         loss = torch.zeros(1, dtype=torch.float32, device=dev)
         total = epochs * nb
-        it = 0
         for ep in range(epochs):
             order = torch.from_numpy(rng.permutation(tr).astype(np.int32)).to(dev)
-            ep_loss = 0.0
-            for b in range(nb):
-                sel = order[b * batch:(b + 1) * batch]
-                _lib.call("altar_da_pack", _lib.ptr(self.theta), _lib.ptr(sel), batch, self.P, _lib.ptr(xb),
-                          _lib.stream())
-                _lib.call("altar_gather_rows_f32", _lib.ptr(target_all), self.K, _lib.ptr(sel), batch, self.K,
-                          _lib.ptr(yb), self.K, _lib.stream())
-                self.mlp.backward(xb, yb, w_d, loss)
-                # cosine decay from lr to lr_final
-                frac = it / max(1, total - 1)
-                cur = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac))
-                self.mlp.adam(cur)
-                it += 1
-            ep_loss = float(loss.item())
-            self.history.append(ep_loss)
+            _lib.call("altar_mlp_train_epoch", self.mlp.handle, _lib.ptr(self.theta), _lib.ptr(target_all), self.K,
+                      _lib.ptr(w_d), _lib.ptr(order), nb, batch, ep * nb, total, float(lr), float(lr_final), 0.9,
+                      0.999, 1e-8, _lib.ptr(loss), _lib.stream())
+            self.history.append(float(loss.item()))
         self.trained = True
         self.val_idx = val_idx
         self.validation_rmse = self.validate_stored(val_idx)

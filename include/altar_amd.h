@@ -267,6 +267,15 @@ int altar_mlp_backward(altar_mlp *h, const double *theta_dev, const float *targe
                        int B, float *loss_dev, void *stream);
 /* one Adam step (Kingma and Ba 2015) with the last gradients */
 int altar_mlp_adam(altar_mlp *h, float lr, float beta1, float beta2, float eps, void *stream);
+/* one epoch in one call: batch b (b < nbatches) takes the rows
+ * order[b batch .. (b + 1) batch) of theta_store (FP64, nin columns) and of
+ * targets (FP32, row pitch ldt >= nout), runs the gradients of
+ * altar_mlp_backward and one Adam step with the cosine-decayed rate
+ * lr_final + (lr - lr_final)(1 + cos(pi it / (total - 1))) / 2, it = it0 + b;
+ * loss_dev (optional) receives the last batch's loss */
+int altar_mlp_train_epoch(altar_mlp *h, const double *theta_store_dev, const float *targets_dev, int ldt,
+                          const float *w_dev, const int *order_dev, int nbatches, int batch, int it0, int total,
+                          float lr, float lr_final, float beta1, float beta2, float eps, float *loss_dev, void *stream);
 /* surrogate data llk from normalized PCA coefficients z (C x ldz):
  * llk = normalization - (cperp + sum_k (e_k + s_k z_k)^2) / 2; e, s device FP64 (K) */
 int altar_surrogate_llk(const float *z_dev, int ldz, int C, int K, const double *e_dev, const double *s_dev,
