@@ -127,22 +127,22 @@ __global__ void mse_grad_kernel(const float *__restrict__ yhat, const float *__r
 }
 
 // db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient), in two passes.
-// Pass 1: block (x, y) sums rows [256 y, 256 y + 256) of columns
-// [64 x, 64 x + 64): lane = column (coalesced rows), 4 waves of 64 rows each,
+// Pass 1: block (x, y) sums rows [64 y, 64 y + 64) of columns
+// [64 x, 64 x + 64): lane = column (coalesced rows), 4 waves of 16 rows each,
 // wave partials added in order into part[y][column].  Pass 2 adds the slabs
 // in order, so the result does not depend on scheduling.
-constexpr int CS_ROWS = 256;
+constexpr int CS_ROWS = 64;              // rows per slab: 4 waves of 16
 __global__ __launch_bounds__(256) void colsum_slab_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld,
                                                           float *__restrict__ part)
 {
     __shared__ float wp[4][64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + lane;
-    const int r0 = blockIdx.y * CS_ROWS + wid * 64;
+    const int r0 = blockIdx.y * CS_ROWS + wid * (CS_ROWS / 4);
     float s = 0.0f;
     if (c < cols) {
 #pragma unroll 8
-        for (int i = 0; i < 64; ++i) {
+        for (int i = 0; i < CS_ROWS / 4; ++i) {
             const int r = r0 + i;
             if (r < rows) s += bf2f(dy[(size_t)r * ld + c]);
         }
