@@ -281,6 +281,41 @@ ALTAR_HD inline constexpr unsigned fp64_bit(int s) { return 1u << (12 + s); }
 // Builds the FP64 geometry and flags of one source.  |sin(beta)| below
 // fp64_sin marks a side for FP64 evaluation: the side coefficients grow
 // like cot(beta)^2 and the FP32 differences across the pair lose accuracy.
+// Side s of the CDM whose vertices are V: its constants into sd (zero when
+// the side is skipped) and its flag bits (skip, FP64, degenerate rectangle).
+ALTAR_HD inline unsigned make_side_of(const double V[12][3], double ax, double ay, double az, double opening,
+                                      double nu, double fp64_sin, int s, Side<double> &sd)
+{
+    sd = Side<double>{};
+    // which rectangles contribute (libcdm.py:226-243, on the full axes)
+    const double Ax = 2.0 * ax, Ay = 2.0 * ay, Az = 2.0 * az;
+    bool on[3] = {true, true, true};
+    if (Ax == 0 && Ay == 0 && Az == 0) { on[0] = on[1] = on[2] = false; }
+    else if (Ax == 0 && Ay != 0 && Az != 0) { on[1] = on[2] = false; }
+    else if (Ax != 0 && Ay == 0 && Az != 0) { on[0] = on[2] = false; }
+    else if (Ax != 0 && Ay != 0 && Az == 0) { on[0] = on[1] = false; }
+    const int q = s / 4;
+    if (!on[q]) return skip_bit(s);
+    const double *P1 = V[4 * q], *P2 = V[4 * q + 1], *P4 = V[4 * q + 3];
+    double e21[3] = {P2[0] - P1[0], P2[1] - P1[1], P2[2] - P1[2]};
+    double e41[3] = {P4[0] - P1[0], P4[1] - P1[1], P4[2] - P1[2]};
+    double vn[3] = {e21[1] * e41[2] - e21[2] * e41[1], e21[2] * e41[0] - e21[0] * e41[2],
+                    e21[0] * e41[1] - e21[1] * e41[0]};
+    double nn = sqrt(vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
+    if (!(nn > 0)) return FLAG_NAN | skip_bit(s);
+    double b[3] = {opening * vn[0] / nn, opening * vn[1] / nn, opening * vn[2] / nn};
+    const double *PA = V[side_vA(s)], *PB = V[side_vB(s)];
+    double sl = (PB[0] - PA[0]) * (PB[0] - PA[0]) + (PB[1] - PA[1]) * (PB[1] - PA[1]) +
+                (PB[2] - PA[2]) * (PB[2] - PA[2]);
+    if (!(sl > 0)) return FLAG_NAN | skip_bit(s);
+    double beta;
+    if (!make_side(PA, PB, b, nu, sd, beta)) {
+        sd = Side<double>{};
+        return skip_bit(s);
+    }
+    return fabs(sin(beta)) < fp64_sin ? fp64_bit(s) : 0u;
+}
+
 ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, double ax, double ay, double az,
                               double omx, double omy, double omz, double opening, double nu,
                               double fp64_sin, Geom<double> &G)
@@ -294,36 +329,7 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
         G.va[v] = -V[v][2];
         G.ka2[v] = 0.5 * (1.0 - 2.0 * nu) * G.va[v];
     }
-    // which rectangles contribute (libcdm.py:226-243, on the full axes)
-    double Ax = 2.0 * ax, Ay = 2.0 * ay, Az = 2.0 * az;
-    bool on[3] = {true, true, true};
-    if (Ax == 0 && Ay == 0 && Az == 0) { on[0] = on[1] = on[2] = false; }
-    else if (Ax == 0 && Ay != 0 && Az != 0) { on[1] = on[2] = false; }
-    else if (Ax != 0 && Ay == 0 && Az != 0) { on[0] = on[2] = false; }
-    else if (Ax != 0 && Ay != 0 && Az == 0) { on[0] = on[1] = false; }
-    for (int q = 0; q < 3; ++q) {
-        const double *P1 = V[4 * q], *P2 = V[4 * q + 1], *P4 = V[4 * q + 3];
-        double e21[3] = {P2[0] - P1[0], P2[1] - P1[1], P2[2] - P1[2]};
-        double e41[3] = {P4[0] - P1[0], P4[1] - P1[1], P4[2] - P1[2]};
-        double vn[3] = {e21[1] * e41[2] - e21[2] * e41[1], e21[2] * e41[0] - e21[0] * e41[2],
-                        e21[0] * e41[1] - e21[1] * e41[0]};
-        double nn = sqrt(vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
-        double b[3] = {opening * vn[0] / nn, opening * vn[1] / nn, opening * vn[2] / nn};
-        for (int k = 0; k < 4; ++k) {
-            int s = 4 * q + k;
-            Side<double> &sd = G.side[s];
-            sd = Side<double>{};
-            if (!on[q]) { flags |= skip_bit(s); continue; }
-            if (!(nn > 0)) { flags |= FLAG_NAN; flags |= skip_bit(s); continue; }
-            double beta;
-            const double *PA = V[side_vA(s)], *PB = V[side_vB(s)];
-            double sl = (PB[0] - PA[0]) * (PB[0] - PA[0]) + (PB[1] - PA[1]) * (PB[1] - PA[1]) +
-                        (PB[2] - PA[2]) * (PB[2] - PA[2]);
-            if (!(sl > 0)) { flags |= FLAG_NAN; flags |= skip_bit(s); continue; }
-            if (!make_side(PA, PB, b, nu, sd, beta)) { flags |= skip_bit(s); continue; }
-            if (fabs(sin(beta)) < fp64_sin) flags |= fp64_bit(s);
-        }
-    }
+    for (int s = 0; s < 12; ++s) flags |= make_side_of(V, ax, ay, az, opening, nu, fp64_sin, s, G.side[s]);
     return flags;
 }
 
@@ -367,6 +373,22 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
     ue = e1x * v1 + e1y * v2;
     un = e1y * v1 - e1x * v2;
     uv = -v3;
+}
+
+template <typename T, typename S>
+ALTAR_HD inline void convert_side(const Side<S> &a, Side<T> &b)
+{
+    b.e1x = (T)a.e1x; b.e1y = (T)a.e1y;
+    b.y1AB = (T)a.y1AB; b.y2AB = (T)a.y2AB;
+    b.b1 = (T)a.b1; b.b2 = (T)a.b2; b.b3 = (T)a.b3;
+    b.s = (T)a.s; b.c = (T)a.c; b.t = (T)a.t;
+    b.hI = (T)a.hI; b.hII = (T)a.hII;
+    b.kt = (T)a.kt; b.e13 = (T)a.e13;
+    for (int j = 0; j < 3; ++j) {
+        b.cF[j] = (T)a.cF[j];
+        b.cL[j] = (T)a.cL[j];
+        b.cZ[j] = (T)a.cZ[j];
+    }
 }
 
 template <typename T, typename S>

@@ -114,28 +114,53 @@ __global__ __launch_bounds__(OBS_TILE) void mogi_kernel(ObsView obs, const doubl
 }
 
 // ------------------------------------------------------------------ CDM
-// Geometry preparation, one thread per chain, FP64 (libcdm.py:163-273).
-__global__ void cdm_prepare_kernel(const double *__restrict__ theta, int C, int P, CdmLayout L,
-                                   double nu, double fp64_sin, cdm::Geom<float> *__restrict__ g32,
-                                   cdm::Geom<double> *__restrict__ g64, unsigned *__restrict__ flags,
-                                   float *__restrict__ shifts /* C x MAX_SETS */)
+// Geometry preparation, FP64 (libcdm.py:163-273), one thread per (chain,
+// side): 16 chains x 12 sides per block.  Each thread builds the vertices
+// (make_vertices) and its side (make_side_of, as make_geometry does for all
+// twelve), and writes that side in FP64 and FP32; the chain's flags are
+// OR-ed in LDS, and the side-0 thread writes the vertices, the dataset
+// shifts and the flags.
+constexpr int PREP_CHAINS = 16;
+__global__ __launch_bounds__(PREP_CHAINS * 12) void cdm_prepare_sides_kernel(
+    const double *__restrict__ theta, int C, int P, CdmLayout L, double nu, double fp64_sin,
+    cdm::Geom<float> *__restrict__ g32, cdm::Geom<double> *__restrict__ g64, unsigned *__restrict__ flags,
+    float *__restrict__ shifts)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    const double *t = theta + (size_t)c * P;
-    cdm::Geom<double> G;
-    unsigned f = cdm::make_geometry(t[L.xIdx], t[L.xIdx + 1], t[L.dIdx], t[L.aXIdx],
-                                        t[L.aXIdx + 1], t[L.aXIdx + 2], t[L.omegaXIdx],
-                                        t[L.omegaXIdx + 1], t[L.omegaXIdx + 2], t[L.openingIdx],
-                                        nu, fp64_sin, G);
-    g64[c] = G;
-    cdm::Geom<float> Gf;
-    cdm::convert(G, Gf);
-    g32[c] = Gf;
-    flags[c] = f;
-    for (int k = 0; k < MAX_SETS; ++k)
-        shifts[(size_t)c * MAX_SETS + k] =
-            (L.offsetIdx >= 0 && k < L.nsets) ? (float)t[L.offsetIdx + k] : 0.0f;
+    __shared__ unsigned sflags[PREP_CHAINS];
+    const int lc = threadIdx.x / 12, s = threadIdx.x % 12;
+    const int c = blockIdx.x * PREP_CHAINS + lc;
+    if (s == 0) sflags[lc] = 0u;
+    __syncthreads();
+    if (c < C) {
+        const double *t = theta + (size_t)c * P;
+        const double ax = t[L.aXIdx], ay = t[L.aXIdx + 1], az = t[L.aXIdx + 2], opening = t[L.openingIdx];
+        double V[12][3];
+        unsigned f = 0;
+        if (cdm::make_vertices(t[L.xIdx], t[L.xIdx + 1], t[L.dIdx], ax, ay, az, t[L.omegaXIdx],
+                               t[L.omegaXIdx + 1], t[L.omegaXIdx + 2], V))
+            f |= cdm::FLAG_INVALID;
+        cdm::Side<double> sd;
+        f |= cdm::make_side_of(V, ax, ay, az, opening, nu, fp64_sin, s, sd);
+        g64[c].side[s] = sd;
+        cdm::Side<float> sf;
+        cdm::convert_side(sd, sf);
+        g32[c].side[s] = sf;
+        atomicOr(&sflags[lc], f);
+        if (s == 0) {
+            for (int v = 0; v < 12; ++v) {
+                const double va = -V[v][2];
+                const double ka2 = 0.5 * (1.0 - 2.0 * nu) * va;
+                g64[c].vx[v] = V[v][0]; g64[c].vy[v] = V[v][1]; g64[c].va[v] = va; g64[c].ka2[v] = ka2;
+                g32[c].vx[v] = (float)V[v][0]; g32[c].vy[v] = (float)V[v][1];
+                g32[c].va[v] = (float)va; g32[c].ka2[v] = (float)ka2;
+            }
+            for (int k = 0; k < MAX_SETS; ++k)
+                shifts[(size_t)c * MAX_SETS + k] =
+                    (L.offsetIdx >= 0 && k < L.nsets) ? (float)t[L.offsetIdx + k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    if (s == 0 && c < C) flags[c] = sflags[lc];
 }
 
 // INLINE64: the FP64 sides are evaluated here; otherwise they are skipped and
@@ -266,8 +291,8 @@ hipError_t launch_mogi(const MogiArgs &a, int mode, int fp64, hipStream_t st)
 
 hipError_t launch_cdm_prepare(const CdmArgs &a, hipStream_t st)
 {
-    dim3 pb(64), pg((a.C + 63) / 64);
-    hipLaunchKernelGGL(cdm_prepare_kernel, pg, pb, 0, st, a.theta, a.C, a.P, a.L, a.nu, a.fp64_sin,
+    dim3 pb(PREP_CHAINS * 12), pg((a.C + PREP_CHAINS - 1) / PREP_CHAINS);
+    hipLaunchKernelGGL(cdm_prepare_sides_kernel, pg, pb, 0, st, a.theta, a.C, a.P, a.L, a.nu, a.fp64_sin,
                        a.g32, a.g64, a.flags, a.shifts);
     return hipGetLastError();
 }
