@@ -164,3 +164,48 @@ def test_joint_fields_and_surrogate(cuda, oracle):
     host_rmse = sur.field_rmse(sur.predict_wr(th), wr2[:, :sur.N].double().cpu().numpy())
     print("field rmse device", dev_rmse, "host", host_rmse)
     assert abs(dev_rmse / host_rmse - 1) < 1e-3
+
+
+def test_joint_fused_trajectory_matches_separate_calls(cuda, oracle):
+    """Surrogate trajectories of the joint source (17 parameters, the CDM part
+    checked against the free surface through the joint layout): the one-launch
+    kernel and the separate calls give the same states, likelihoods and
+    counters bit for bit."""
+    print("case: joint fused vs separate trajectories, 2 x 400 InSAR + 30 GNSS, C=700, 2 x 5 sub-steps", flush=True)
+    from altar_amd.bayesian import Annealer, COV, Metropolis
+    from altar_amd.bayesian.da import DelayedAcceptance
+    from altar_amd.surrogate import Surrogate
+    case = cases.joint_case(n_insar=400, n_gnss=30, forward=oracle_forward(oracle))
+    C = 700
+    job = Job(chains=C, steps=1, seed=8)
+    model = Joint(case["psets"], case["datasets"], job=job).initialize()
+    ann = Annealer(Metropolis(), COV(), seed=8).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    w.cool(ann)
+    step = w.step
+    ann.sampler.prepareSamplingPDF(ann, step)
+    P = step.parameters
+    sur = Surrogate(model, K=32, capacity=1024, max_batch=C, seed=2)
+    rng = np.random.default_rng(5)
+    sur.e_d = torch.from_numpy(rng.normal(0.0, 1.0, 32)).to(cuda)
+    sur.s_d = torch.from_numpy(rng.uniform(0.05, 0.2, 32)).to(cuda)
+    sur.cperp = 2.0
+    sur.trained = True
+    runs = []
+    for fused in (False, True):
+        da = DelayedAcceptance(sur, steps=2, substeps=5, substep_scale=2.5, fused=fused)
+        assert da.fused_supported(model, P) == fused
+        da.sigma_chol = ann.sampler.sigma_chol
+        st = step.clone()
+        stats = da._walk_trajectory(ann, st)
+        torch.cuda.synchronize()
+        runs.append((stats, st, list(da.substep_counts), {k: v.clone() for k, v in da._dws.items()}))
+    (s0, st0, c0, w0), (s1, st1, c1, w1) = runs
+    print("exact-stage stats", s0, s1, "sub-steps", c0, c1)
+    assert s0 == s1 and c0 == c1 and c0[0] > 0 and c0[1] > 0
+    for name in ("theta", "prior", "data", "posterior"):
+        assert torch.equal(getattr(st0, name), getattr(st1, name)), name
+    for name in ("phi", "prior_phi", "post_phi", "post_s"):
+        assert torch.equal(w0[name], w1[name]), name
