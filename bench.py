@@ -503,6 +503,23 @@ def fused_kernel_timing(ann, da, start, subs, reps=3):
             "network_flops_per_launch": flops, "substeps": subs}
 
 
+def surrogate_llk_error(ann, sur, step, n):
+    """Surrogate against exact data llk on the first n chains of the stage:
+    the spread of the difference, times beta, is what the exact stage of
+    delayed acceptance sees (its acceptance falls as beta sd grows)."""
+    import torch
+    from altar_amd import _lib
+    theta = step.theta[:n].contiguous()
+    exact = torch.empty(n, dtype=torch.float64, device=theta.device)
+    surr = torch.empty(n, dtype=torch.float64, device=theta.device)
+    model = ann.model
+    model.source.data_llk(model.restrict(theta), exact, model.precision)
+    sur.data_llk(theta, surr)
+    d = (exact - surr).cpu().numpy()
+    return {"chains": n, "beta": float(step.beta), "mean_nats": float(d.mean()), "sd_nats": float(d.std()),
+            "beta_times_sd": float(step.beta * d.std())}
+
+
 def moment_check(ann, da, plain, start, steps, subs, reps=4, bias_sd=1.0, seed=41, equilibrate=100):
     """Does the exact stage correct the surrogate's error?  Posterior means
     at one temperature by trajectory delayed acceptance with the trained
@@ -629,6 +646,7 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     step = w.step
     da.prepareSamplingPDF(ann, step)
     ok = da._validate(ann, step)
+    llk_err = surrogate_llk_error(ann, sur, step, n=min(2048, C)) if sur.trained else None
     start = step.clone()
     sd = start.theta.std(0)
     P = start.theta.shape[1]
@@ -720,7 +738,7 @@ def da_leg(args, model, C, seed=777 + 3, capacity=None, stage_steps=5, fit_kwarg
     out = {
         "beta": step.beta, "steps": K, "validation_rmse_m": sur.validation_rmse, "surrogate_used": bool(ok),
         "refits": da.refits, "modes": [m["mode"] for m in da.mode_history],
-        "training_fields": sur.count, "fit_s": t_fit, "setup_s": setup,
+        "training_fields": sur.count, "fit_s": t_fit, "setup_s": setup, "surrogate_llk_error": llk_err,
         "mlp_flops_per_chain": 2 * (model.parameters * 256 + 256 * 256 * 2 + 256 * 64),
     }
     out.update(res)
