@@ -76,7 +76,6 @@ SIGNATURES = {
     "altar_resample_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "altar_posterior": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "altar_random_normals": (_i, [_vp, _i, _i, _u64, _u32, _u32, _u32, _vp]),
-    "altar_valu_probe": (_i, [_i, _vp]),
     "altar_gemm_f32": (_i, [_vp, _sz, _vp, _sz, _i, _i, _i, _vp, _vp, _sz, _vp]),
     "altar_gather_rows_f32": (_i, [_vp, _sz, _vp, _i, _i, _vp, _sz, _vp]),
     "altar_transpose_f32": (_i, [_vp, _sz, _i, _i, _vp, _sz, _vp]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaltar_amd.so")
-SOURCES = ["sources.hip", "gemm.hip", "sampler.hip", "mlp.hip", "cov.hip", "trajectory.hip", "capi.hip", "probe.hip"]
+SOURCES = ["sources.hip", "gemm.hip", "sampler.hip", "mlp.hip", "cov.hip", "trajectory.hip", "capi.hip"]
 ARCH = os.environ.get("ALTAR_OFFLOAD_ARCH", "gfx950")
 
 

@@ -88,8 +88,6 @@ def parse():
                     help="exact steps per replicate of the posterior-moment comparison (0 skips it)")
     ap.add_argument("--da-substep-target", type=float, default=0.3,
                     help="sub-step acceptance rate the pilot trajectories tune the sub-step scale to (0: fixed)")
-    ap.add_argument("--valu-calibration", action="store_true",
-                    help="measure the VALU issue rates of FMA, multiply, add, reciprocal and FP64 FMA")
     ap.add_argument("--trajectory-kernel", type=int, default=0, metavar="SUBSTEPS",
                     help="only time altar_da_trajectory_fused on the CDM chains (SUBSTEPS sub-steps, network "
                          "with its initial weights), for counter runs")
@@ -109,22 +107,6 @@ def copy_calibration(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
-
-
-def valu_calibration():
-    """Measured VALU issue rates (lane-operations per cycle per SIMD at the
-    2.4 GHz peak clock) of the instruction kinds of the forward kernels."""
-    import ctypes
-    import torch
-    from altar_amd import _lib
-    torch.cuda.init()
-    names = ["fp32_fma", "fp32_fma_uniform", "fp32_mul", "fp32_add", "fp32_rcp", "fp64_fma"]
-    out = {}
-    for k, name in enumerate(names):
-        r = ctypes.c_double()
-        _lib.call("altar_valu_probe", k, ctypes.byref(r))
-        out[name] = {"lane_ops_per_s": r.value, "per_clk_per_simd": r.value / (1024 * 2.4e9)}
-    print(json.dumps({"valu_calibration": out}))
 
 
 def trajectory_kernel(args, subs, reps=3):
@@ -183,11 +165,32 @@ def trajectory_kernel(args, subs, reps=3):
                                             "substep_counts": cnt.tolist()}}))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+_BASELINE_DIR = None
+
+
 def cpu_baseline(case, theta, seconds, omp=False):
     """FP64 oracle (oracle/oracle.c): CDM residuals + whitened L2 on a bounded
-    sample of the chains; one thread, or the OpenMP build over chains."""
+    sample of the chains; one thread, or OpenMP over chains.  Both builds are
+    compiled here, on the host that runs them, with gcc -O3 -march=native."""
+    global _BASELINE_DIR
+    import tempfile
     from oracle import oracle as o
     from altar_amd import norms
+    if _BASELINE_DIR is None:
+        _BASELINE_DIR = tempfile.mkdtemp(prefix="altar_oracle_")
+        o.baseline_build(_BASELINE_DIR)
+    variant = "native_omp" if omp else "native"
     layout = (0, 2, 3, 4, 7, -1)
     W = o.exact_whitener(case["cd"])
     norm = norms.normalization(case["cd"])
@@ -196,8 +199,8 @@ def cpu_baseline(case, theta, seconds, omp=False):
 
     def run(n):
         t0 = time.perf_counter()
-        r = o.cdm_residuals(theta[:n], layout, *args, omp=omp)
-        o.l2_llk(r, W, norm, omp=omp)
+        r = o.cdm_residuals(theta[:n], layout, *args, omp=variant)
+        o.l2_llk(r, W, norm, omp=variant)
         return time.perf_counter() - t0
 
     run(threads)                       # warm-up (thread pool, caches)
@@ -206,6 +209,7 @@ def cpu_baseline(case, theta, seconds, omp=False):
     n = int(max(1, min(len(theta), seconds / max(t1, 1e-6) * n1)))
     t = run(n)
     return {"value": n / t, "unit": "likelihood evals/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "build": "gcc -O3 -march=native -fno-fast-math",
             "sample": f"{n} chains x {len(case['x'])} obs, FP64 CDM + full-Cd whitening, "
                       f"{threads} thread{'s' if threads > 1 else ''}, {t:.1f} s"}
 
@@ -214,9 +218,6 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
-        return 0
-    if args.valu_calibration:
-        valu_calibration()
         return 0
     if args.trajectory_kernel:
         trajectory_kernel(args, args.trajectory_kernel)

@@ -345,16 +345,8 @@ int altar_da_trajectory_fused(altar_mlp *mlp, const altar_prior *prior, const al
 /* index_dev[0..count) = the chains whose trajectory moved, in chain order */
 int altar_da_trajectory_list(int *index_dev, int C, int *count_dev, void *stream);
 
-/* Philox normals and uniforms into device memory (the same streams the
- * kernels draw), for parity checks */
-/* VALU throughput calibration (no reference counterpart): the sustained
- * lane-operations per second of one instruction kind over the whole device,
- * 8 independent chains per thread.  kind: 0 FP32 FMA (per-lane operands),
- * 1 FP32 FMA (wave-uniform operands), 2 FP32 multiply, 3 FP32 add,
- * 4 FP32 hardware reciprocal, 5 FP64 FMA.  Allocates and frees its own
- * buffers and synchronizes the device. */
-int altar_valu_probe(int kind, double *lane_ops_per_s);
-
+/* Philox normals into device memory (the same streams the kernels draw),
+ * for parity checks */
 int altar_random_normals(double *out_dev, int C, int P, uint64_t seed, uint32_t stage, uint32_t step,
                          uint32_t tag, void *stream);
 

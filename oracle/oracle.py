@@ -19,8 +19,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
-_LIB_OMP = None
+_LIBS = {}
+# where the host-tuned baseline builds go (set by baseline_build)
+_NATIVE_DIR = None
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
@@ -30,6 +31,19 @@ _u32 = ctypes.POINTER(ctypes.c_uint32)
 def build():
     """Compile liboracle.so (and the OpenMP baseline variant) in place."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def baseline_build(outdir):
+    """Compile the CPU-baseline builds for the host that runs them:
+    gcc -O3 -march=native, one thread (liboracle_native.so) and OpenMP over
+    chains (liboracle_native_omp.so), into outdir.  Afterwards the omp
+    argument of the forward and likelihood wrappers selects "native" or
+    "native_omp" as well."""
+    global _NATIVE_DIR
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"OUTDIR={os.path.abspath(outdir)}"], check=True)
+    _NATIVE_DIR = os.path.abspath(outdir)
+    for k in ("native", "native_omp"):
+        _LIBS.pop(k, None)
 
 
 def _declare(lib):
@@ -66,19 +80,21 @@ def _declare(lib):
 
 
 def lib(omp=False):
-    """The oracle library (``omp=True``: the -O3 OpenMP build for the CPU baseline)."""
-    global _LIB, _LIB_OMP
-    name = "liboracle_omp.so" if omp else "liboracle.so"
-    path = os.path.join(_HERE, name)
-    if not os.path.exists(path):
-        build()
-    if omp:
-        if _LIB_OMP is None:
-            _LIB_OMP = _declare(ctypes.CDLL(path))
-        return _LIB_OMP
-    if _LIB is None:
-        _LIB = _declare(ctypes.CDLL(path))
-    return _LIB
+    """The oracle library.  omp=False: the test build; True: the -O3 OpenMP
+    build; "native" / "native_omp": the -O3 -march=native builds made by
+    baseline_build for the CPU baseline."""
+    key = {False: "plain", True: "omp"}.get(omp, omp)
+    if key not in _LIBS:
+        if key in ("native", "native_omp"):
+            if _NATIVE_DIR is None:
+                raise RuntimeError("baseline_build(outdir) has not run")
+            path = os.path.join(_NATIVE_DIR, f"liboracle_{key}.so")
+        else:
+            path = os.path.join(_HERE, "liboracle_omp.so" if key == "omp" else "liboracle.so")
+            if not os.path.exists(path):
+                build()
+        _LIBS[key] = _declare(ctypes.CDLL(path))
+    return _LIBS[key]
 
 
 def _p(a, t=_d):

@@ -147,6 +147,81 @@ def test_dbeta_grid_against_scan(oracle):
     assert b1 == 1.0
 
 
+def test_cov_metric_against_independent_forms(oracle):
+    """oracle_cov_eval (COV.cc:334-368): w = exp(dbeta (llk - median)) / sum,
+    COV = sd_{n-1}(w) / mean(w), metric (COV - target)^2, and 1e100 when the
+    COV is not finite.  Pinned by (i) scipy's softmax and numpy's moments, a
+    route that never subtracts the median (the COV does not depend on it);
+    (ii) a two-valued llk, whose COV has a closed form; (iii) the lognormal
+    limit sqrt(exp((dbeta s)^2) - 1) for Gaussian llk; (iv) dbeta = 0, where
+    every weight is 1/C and the COV is 0.  A sign flip of dbeta (llk - median)
+    fails (ii), variance in place of sd fails (ii) and (iii), a population sd
+    fails (ii) at small C."""
+    from scipy.special import softmax
+    rng = np.random.default_rng(12)
+    for C, dbeta, scale in ((7, 0.3, 2.0), (1000, 0.01, 40.0), (4096, 0.05, 15.0), (333, 1.7, 0.5)):
+        llk = rng.normal(-500.0, scale, C)
+        med = oracle.median(llk)
+        metric, cov, w = oracle.cov_eval(dbeta, llk, med, target=1.0)
+        wn = softmax(dbeta * llk)
+        want = np.std(wn, ddof=1) / np.mean(wn)
+        assert np.allclose(w, wn, rtol=1e-11, atol=0)
+        assert abs(cov - want) <= 1e-11 * want
+        assert metric == pytest.approx((want - 1.0) ** 2, rel=1e-10, abs=1e-24)
+        # the target enters only the metric
+        m2, c2, _ = oracle.cov_eval(dbeta, llk, med, target=0.25)
+        assert c2 == cov and m2 == pytest.approx((want - 0.25) ** 2, rel=1e-10)
+    # (ii) k samples at llk = a, C - k at llk = b: weights take two values
+    # u = e^{dbeta a}, v = e^{dbeta b}; with m = (k u + (C-k) v) / C,
+    # var_{n-1} = (k (u-m)^2 + (C-k) (v-m)^2) / (C - 1)
+    for C, k, a, b, dbeta in ((5, 2, -3.0, -1.0, 0.7), (101, 7, 10.0, 2.0, 0.2), (64, 60, 0.0, 4.0, 1.1)):
+        llk = np.array([a] * k + [b] * (C - k))
+        u, v = np.exp(dbeta * a), np.exp(dbeta * b)
+        m = (k * u + (C - k) * v) / C
+        want = np.sqrt((k * (u - m) ** 2 + (C - k) * (v - m) ** 2) / (C - 1)) / m
+        _, cov, w = oracle.cov_eval(dbeta, rng.permutation(llk), oracle.median(llk))
+        assert abs(cov - want) <= 1e-12 * want
+        assert np.isclose(w.sum(), 1.0, rtol=1e-14)
+        # the median only rescales the unnormalized weights
+        _, cov2, _ = oracle.cov_eval(dbeta, llk, oracle.median(llk) + 3.0)
+        assert abs(cov2 - want) <= 1e-12 * want
+    # (iii) lognormal weights: COV -> sqrt(exp(t^2) - 1), t = dbeta s; the
+    # sample COV's sd is about 4% here at C = 1e6, so 5 of them bound it
+    C = 1_000_000
+    llk = rng.normal(-2000.0, 30.0, C)
+    for t in (0.1, 0.4, 0.7):
+        _, cov, _ = oracle.cov_eval(t / 30.0, llk, oracle.median(llk))
+        want = np.sqrt(np.expm1(t * t))
+        assert abs(cov / want - 1.0) < 0.02 + 0.03 * t * t
+    # (iv) dbeta = 0: equal weights (the sd is rounding only)
+    metric, cov, w = oracle.cov_eval(0.0, llk[:1000], oracle.median(llk[:1000]))
+    assert cov < 1e-14 and abs(metric - 1.0) < 1e-13 and np.all(w == w[0]) and np.isclose(w[0], 1e-3, rtol=1e-15)
+    # not finite: an exponent past the double range gives inf weights, then a
+    # NaN COV; the reference returns the metric and the COV as 1e100
+    metric, cov, _ = oracle.cov_eval(1.0, np.array([0.0, 0.0, 1e5, -1e5]), 0.0)
+    assert metric == 1e100 and cov == 1e100
+
+
+def test_mh_scaling_special_cases(oracle):
+    """Metropolis.adjustCovarianceScaling (Metropolis.py:262-281):
+    kc = (aw acc + rw) / (aw + rw) clamped to [0.1, 1], acc = A / (A + R + U).
+    Special cases fix each term: everything accepted gives exactly 1, nothing
+    accepted gives rw / (aw + rw) = 1/9 at the default weights (above the
+    clamp); weights that push kc below 0.1 or above 1 are clamped; rejected
+    and unlikely moves count alike."""
+    assert oracle.mh_adjust_scaling(1000, 0, 0) == 1.0
+    assert oracle.mh_adjust_scaling(0, 500, 500) == pytest.approx(1.0 / 9.0, rel=1e-15)
+    assert oracle.mh_adjust_scaling(0, 1000, 0) == oracle.mh_adjust_scaling(0, 0, 1000)
+    assert oracle.mh_adjust_scaling(25, 75, 0) == pytest.approx((8 * 0.25 + 1) / 9, rel=1e-15)
+    assert oracle.mh_adjust_scaling(25, 0, 75) == oracle.mh_adjust_scaling(25, 75, 0)
+    # weights of 1 and 0: kc = acc, clamped below at 0.1
+    assert oracle.mh_adjust_scaling(5, 95, 0, aw=1.0, rw=0.0) == 0.1
+    assert oracle.mh_adjust_scaling(30, 70, 0, aw=1.0, rw=0.0) == pytest.approx(0.3, rel=1e-15)
+    # a negative rejection weight can push kc past 1: clamped above
+    assert oracle.mh_adjust_scaling(100, 0, 0, aw=2.0, rw=-0.5) == 1.0
+    assert oracle.mh_adjust_scaling(50, 50, 0, aw=2.0, rw=-0.5) == pytest.approx(0.5 / 1.5, rel=1e-15)
+
+
 def test_weighted_covariance_and_multiplicities(oracle):
     rng = np.random.default_rng(3)
     theta = rng.normal(size=(500, 4))
