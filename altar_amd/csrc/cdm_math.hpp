@@ -104,6 +104,10 @@ ALTAR_HD inline float f_atan2(float y, float x)
 // random sources against the FP64 oracle); the logarithms and reciprocals,
 // whose coefficients grow like cot(beta)^2, stay in FP64.
 ALTAR_HD inline double f_atan2(double y, double x) { return (double)f_atan2((float)y, (float)x); }
+// The Burgers angle difference of a side: f_atan2, or with exact set (every
+// side in FP64, the reference's precision) the FP64 atan2.
+ALTAR_HD inline float side_angle(float y, float x, bool) { return f_atan2(y, x); }
+ALTAR_HD inline double side_angle(double y, double x, bool exact) { return exact ? atan2(y, x) : f_atan2(y, x); }
 
 // Side constants of one angular-dislocation pair.  Configuration II uses
 // the angle beta and configuration I the angle beta - pi (libcdm.py:344-362):
@@ -347,7 +351,7 @@ template <typename T>
 ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
                                T dxA, T dyA, T aA, T kaA, T rA, T irA, T irpaA,
                                T aB, T kaB, T rB, T irB, T irpaB,
-                               T &ue, T &un, T &uv)
+                               T &ue, T &un, T &uv, bool exact_angle = false)
 {
     const T e1x = sd.e1x, e1y = sd.e1y;
     const T y1A = e1x * dxA + e1y * dyA;
@@ -366,7 +370,7 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
                     v1A, v2A, v3A, pA, qA, fyA, fxA, rpaA);
     vertex_terms<T>(y1B, y2B, aB, kaB, rB, irB, irpaB, s, c, h, e13, sd, k05, nu2,
                     v1B, v2B, v3B, pB, qB, fyB, fxB, rpaB);
-    const T dFi = f_atan2(fyB * fxA - fyA * fxB, fxA * fxB + fyA * fyB);
+    const T dFi = side_angle(fyB * fxA - fyA * fxB, fxA * fxB + fyA * fyB, exact_angle);
     // log((rB + aB) / (rA + aA)) and log((rB - z3B) / (rA - z3A))
     const T dLr = f_log(rpaB * irpaA);
     const T dLz = sg * f_log(f_div(pB * qA, qB * pA));

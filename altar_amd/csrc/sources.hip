@@ -171,7 +171,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                                                        const cdm::Geom<double> *__restrict__ g64,
                                                        const unsigned *__restrict__ flags,
                                                        const float *__restrict__ shifts, int nsets,
-                                                       float nu, Out out)
+                                                       float nu, int exact64, Out out)
 {
     const int chain = blockIdx.x;
     const int n = blockIdx.y * OBS_TILE + threadIdx.x;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                         double e, nn, u;
                         cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya, aa,
                                                H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa), ab, H.ka2[b], rb,
-                                               cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
+                                               cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u, exact64 != 0);
                         due += e; dun += nn; duv += u;
                     } else {
                         float e, nn, u;
@@ -344,7 +344,8 @@ template <int MODE>
 __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const cdm::Geom<double> *__restrict__ g64,
                                                             const unsigned *__restrict__ flags,
                                                             const int *__restrict__ list,
-                                                            const int *__restrict__ count, double nu, Out out)
+                                                            const int *__restrict__ count, double nu,
+                                                            int exact64, Out out)
 {
     const int n = blockIdx.x * OBS_TILE + threadIdx.x;
     const int cnt = *count;
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const c
             double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
             double e, nn, u;
             cdm::side_disp<double>(H.side[s], nu, k, dxa, dya, aa, H.ka2[a], ra, cdm::f_rcp(ra), cdm::f_rcp(ra + aa),
-                                   ab, H.ka2[b], rb, cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u);
+                                   ab, H.ka2[b], rb, cdm::f_rcp(rb), cdm::f_rcp(rb + ab), e, nn, u, exact64 != 0);
             due += e; dun += nn; duv += u;
         }
         if (n < obs.N) {
@@ -386,12 +387,15 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
 {
     dim3 grid(a.C, (a.obs.N + OBS_TILE - 1) / OBS_TILE), block(OBS_TILE);
     float nu = (float)a.nu;
+    // every side in FP64 (threshold above 1): the reference's precision,
+    // with the FP64 atan2 for the Burgers angle
+    const int exact64 = a.fp64_sin > 1.0 ? 1 : 0;
     // residual modes take the two-pass path (FP32 sides, then the FP64 sides
     // of the chains that have any); the partial-sum and BF16 epilogues need
     // the complete residual in one kernel
     const bool split = (mode == MODE_RES_F32 || mode == MODE_RES_F64 || mode == MODE_ACC_F32 ||
                         mode == MODE_ACC_F64) && a.fp64_list != nullptr && a.fp64_count != nullptr;
-#define CDM_LAUNCH(M, I) hipLaunchKernelGGL((cdm_kernel<M, I>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, a.out)
+#define CDM_LAUNCH(M, I) hipLaunchKernelGGL((cdm_kernel<M, I>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, exact64, a.out)
     if (!split) {
         if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG, true);
         else if (mode == MODE_RES_F32) CDM_LAUNCH(MODE_RES_F32, true);
@@ -415,10 +419,10 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
     dim3 g2((a.obs.N + OBS_TILE - 1) / OBS_TILE, (unsigned)std::min(a.C, 2048));
     if (mode == MODE_RES_F32 || mode == MODE_ACC_F32)
         hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F32>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
-                           a.fp64_count, a.nu, a.out);
+                           a.fp64_count, a.nu, exact64, a.out);
     else
         hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F64>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
-                           a.fp64_count, a.nu, a.out);
+                           a.fp64_count, a.nu, exact64, a.out);
     return hipGetLastError();
 }
 

@@ -6,9 +6,11 @@
 * Linear, G 4,000 x 400, full Cd (pre-whitened GEMM): FP32, BF16 +
   refinement and plain BF16.
 
-Bounds: CDM displacements 2e-4 of the field max; llk relative 1e-4 (FP32),
-1e-3 (BF16 + refinement); linear llk relative 1e-5 (FP32), 1e-4 (BF16 +
-refinement).  Plain BF16 is reported, not bounded.  The measured figures
+Bounds, 3-5 times the largest error measured on an MI355X: CDM
+displacements 5e-5 of the field max (measured 1.3e-5); llk relative 4e-6
+(FP32, measured 1.0e-6), 1.2e-3 (BF16 + refinement, 3.5e-4); linear llk
+relative 3e-7 (FP32, 8.2e-8), 4e-6 (BF16 + refinement, 1.1e-6).  Plain BF16
+is reported, not bounded.  The measured figures
 are printed as one JSON line starting with PRECISION.
 """
 import json
@@ -74,9 +76,9 @@ def test_precision_table(cuda, oracle):
         model.source.data_llk(t, llk, "fp32")
         d = np.abs(llk.cpu().numpy() - ref)
         out[f"cdm_llk_fp32_fp64sides{thr}"] = {"abs": stats(d), "abs_near_truth": stats(d[64:])}
-        assert fe.max() < 2e-4
+        assert fe.max() < 5e-5
     model.source.fp64_sides(model.fp64_sides)
-    for prec, tol in (("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
+    for prec, tol in (("fp32", 4e-6), ("bf16r", 1.2e-3), ("bf16", None)):
         llk = torch.empty(C, dtype=torch.float64, device=cuda)
         model.source.data_llk(t, llk, prec)
         d = np.abs(llk.cpu().numpy() - ref)
@@ -93,7 +95,7 @@ def test_precision_table(cuda, oracle):
     lref = oracle.l2_llk(oracle.linear_residuals(th, lc["G"], lc["d"]), oracle.exact_whitener(lc["cd"]),
                          oracle.normalization(lc["cd"]))
     tl = torch.from_numpy(th).to(cuda)
-    for prec, tol in (("fp32", 1e-5), ("bf16r", 1e-4), ("bf16", None)):
+    for prec, tol in (("fp32", 3e-7), ("bf16r", 4e-6), ("bf16", None)):
         m = Linear(lc["psets"], lc["G"], lc["d"], lc["cd"], precision=prec, job=Job(chains=Cl)).initialize()
         llk = torch.empty(Cl, dtype=torch.float64, device=cuda)
         m.source.data_llk(tl, llk, prec)

@@ -1,13 +1,17 @@
 """GPU parity of the forward models and the data log-likelihood against the
 FP64 oracle (oracle/oracle.c), on identical theta and observations.
 
-Tolerances (stated per test):
-* Mogi FP64 path: residuals and llk to 1e-12 relative.
-* Mogi FP32 path: residuals to 1e-5 of the field's max; llk to 1e-5 relative.
-* CDM (FP32, FP64 near-vertical sides): displacement error <= 2e-4 of the
-  field's max per sample (normwise); llk relative error <= 1e-4.
-* Whitening, full Cd: FP32 MFMA llk rel. error <= 1e-4; BF16 + refinement
-  <= 1e-3; plain BF16 is reported, not bounded.
+Each bar is 3-5 times the largest error measured on an MI355X (stated per
+test as "measured"):
+* Mogi FP64 path: residuals to 5e-15 of the field's max, llk to 2e-13 relative.
+* Mogi FP32 path: residuals to 3e-6 of the field's max; llk to 3e-5 relative.
+* CDM (FP32, FP64 near-vertical sides): displacement error <= 1e-4 of the
+  field's max per sample (normwise), golden vectors <= 1.2e-4; llk relative
+  error <= 6e-6 (FP32 whitening), 6e-5 (BF16 + refinement).
+* CDM with every side in FP64: the reference's own bar, 1e-6 relative per
+  component at its MATLAB points (models/cdm/tests/libcdm.py:83-86).
+* Whitening, full Cd (Mogi): FP64 2e-13, FP32 MFMA 3e-5, BF16 + refinement
+  5e-4 relative; plain BF16 is reported, not bounded.
 """
 import numpy as np
 import pytest
@@ -60,8 +64,10 @@ def test_mogi_residuals_fp64_fp32(cuda, oracle):
     r64 = src.residuals(t, "fp64").cpu().numpy()
     r32 = src.residuals(t, "fp32").cpu().numpy()
     scale = np.abs(ref).max(axis=1, keepdims=True)
-    assert np.max(np.abs(r64 - ref) / scale) < 1e-12
-    assert np.max(np.abs(r32 - ref) / scale) < 1e-5
+    e64, e32 = np.max(np.abs(r64 - ref) / scale), np.max(np.abs(r32 - ref) / scale)
+    print("mogi residual err of the field max: fp64", e64, "fp32", e32)
+    assert e64 < 5e-15     # measured 1.1e-15
+    assert e32 < 3e-6      # measured 6.5e-7
 
 
 def test_mogi_llk_diag(cuda, oracle):
@@ -78,7 +84,8 @@ def test_mogi_llk_diag(cuda, oracle):
     src = make_mogi(case, 5)
     src.whitening("diag", W, norm)
     t = torch.from_numpy(theta).to(cuda)
-    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+    # measured: fp64 4.6e-14, fp32 8.4e-6
+    for prec, tol in (("fp64", 2e-13), ("fp32", 3e-5)):
         out = torch.empty(1024, dtype=torch.float64, device=cuda)
         src.data_llk(t, out, prec)
         got = out.cpu().numpy()
@@ -111,7 +118,8 @@ def test_mogi_llk_full_cd(cuda, oracle, N):
             q = np.einsum("ci,ci->c", res, np.linalg.solve(cd, res.T).T)
             assert np.allclose(ref, norm - q / 2, rtol=1e-9)
         src.whitening(kind, W, norm)
-        for prec, tol in (("fp64", 1e-10), ("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
+        # measured (both modes, N = 300 and 1,000): fp64 4.7e-14, fp32 7.8e-6, bf16r 1.5e-4
+        for prec, tol in (("fp64", 2e-13), ("fp32", 3e-5), ("bf16r", 5e-4), ("bf16", None)):
             out = torch.empty(512, dtype=torch.float64, device=cuda)
             src.data_llk(t, out, prec)
             got = out.cpu().numpy()
@@ -137,7 +145,7 @@ def test_cdm_forward_vs_oracle(cuda, oracle):
     got = src.forward(torch.from_numpy(theta).to(cuda), "fp32").cpu().numpy()
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     print("cdm normwise err: median", np.median(err), "max", err.max())
-    assert err.max() < 2e-4
+    assert err.max() < 1e-4     # measured 2.4e-5
 
 
 def test_cdm_golden_vectors(cuda):
@@ -160,8 +168,36 @@ def test_cdm_golden_vectors(cuda):
             scale = max(np.abs(g["ue"][k]).max(), np.abs(g["un"][k]).max(), np.abs(g["uv"][k]).max())
             e = np.abs(got - ref).max() / scale
             worst = max(worst, e)
-            assert e < 2e-4, (k, comp, e)
+            assert e < 1.2e-4, (k, comp, e)     # measured worst 3.5e-5
     print("golden worst normwise", worst)
+
+
+def test_cdm_every_side_fp64_matlab_points(cuda):
+    """Every side in FP64 (threshold 2, with the FP64 atan2): the reference's
+    MATLAB points (models/cdm/tests/libcdm.py:27-46, parameters :103-117) to
+    its own bar, 1e-6 relative per component (:83-86); the FP32 path's
+    component errors are printed beside it."""
+    from test_oracle_pins import KAT_DE, KAT_DN, KAT_DV, KAT_PARAMS, KAT_X, KAT_Y
+    p = KAT_PARAMS
+    M = len(KAT_X)
+    th = torch.tensor([[p["X0"], p["Y0"], p["depth"], p["opening"], p["ax"], p["ay"], p["az"], p["omegaX"],
+                        p["omegaY"], p["omegaZ"]]], dtype=torch.float64, device=cuda)
+    worst = {}
+    for thr in (2.0, 0.2):
+        w = 0.0
+        for los, want in (([1, 0, 0], KAT_DE), ([0, 1, 0], KAT_DN), ([0, 0, 1], KAT_DV)):
+            src = Source(_lib.CDM, 10, M, p["nu"])
+            src.layout([0, 2, 3, 4, 7, -1, 0])
+            src.fp64_sides(thr)
+            src.observations(KAT_X, KAT_Y, np.tile(los, (M, 1)), np.zeros(M), np.zeros(M, np.int32))
+            got = src.forward(th, "fp32").cpu().numpy()[0]
+            rel = np.abs((got - want) / want)
+            w = max(w, rel.max())
+            if thr == 2.0:
+                assert rel.max() < 1e-6, (los, rel.max())
+        worst[thr] = w
+    print("MATLAB points, max relative error per component: every side FP64", worst[2.0],
+          "| default threshold 0.2", worst[0.2])
 
 
 def test_cdm_verify_matches_vertices(cuda, oracle):
@@ -202,7 +238,8 @@ def test_cdm_llk_full_cd(cuda, oracle):
     src.observations(x, y, los, d, np.zeros(N, np.int32))
     src.whitening(kind, W, norm)
     t = torch.from_numpy(theta).to(cuda)
-    for prec, tol in (("fp32", 1e-4), ("bf16r", 1e-3), ("bf16", None)):
+    # measured: fp32 1.8e-6, bf16r 1.6e-5
+    for prec, tol in (("fp32", 6e-6), ("bf16r", 6e-5), ("bf16", None)):
         out = torch.empty(128, dtype=torch.float64, device=cuda)
         src.data_llk(t, out, prec)
         got = out.cpu().numpy()
@@ -250,8 +287,8 @@ def test_fields_write_only_the_callers_rows(cuda, oracle):
 
 def test_cdm_fp64_side_policy(cuda, oracle):
     """The FP64-side threshold: 0 puts every side in FP32, 2 every side in
-    FP64 (then only the final FP32 rounding remains: <= 2e-6 normwise); the
-    census counts the sides of each kind."""
+    FP64 with the FP64 atan2 (<= 1e-10 normwise); the census counts the
+    sides of each kind."""
     print("case: cdm fp64 side policy C=48 N=1500", flush=True)
     rng = np.random.default_rng(14)
     N = 1500
@@ -276,7 +313,7 @@ def test_cdm_fp64_side_policy(cuda, oracle):
     assert census[0.0][3] == 0 and sum(census[0.0][:3]) == 12 * 48
     for thr in (0.4, 2.0):
         assert sum(census[thr][:3]) == 12 * 48 and census[thr][2] == census[0.0][2]
-    assert errs[2.0] < 2e-6
-    assert errs[0.4] < 2e-4
+    assert errs[2.0] < 1e-10    # measured 2.4e-11
+    assert errs[0.4] < 2e-5     # measured 3.5e-6
     with pytest.raises(RuntimeError):
         src.fp64_sides(-1.0)
