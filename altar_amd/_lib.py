@@ -73,6 +73,7 @@ SIGNATURES = {
     "altar_cov_multiplicities": (_i, [_vp, _i, _u64, _u32, _vp]),
     "altar_cov_update_dev": (_i, [_vp, _i, _d, _d, _d, _vp, _vp, _vp]),
     "altar_cov_resample_dev": (_i, [_vp, _i, _u64, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "altar_cov_resample_dev_noalloc": (_i, [_vp, _i, _u64, _u32, _vp, _vp, _vp]),
     "altar_resample_gather": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "altar_posterior": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "altar_random_normals": (_i, [_vp, _i, _i, _u64, _u32, _u32, _u32, _vp]),

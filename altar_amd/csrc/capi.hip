@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -1699,6 +1700,65 @@ int altar_cov_resample_dev(const double *w_dev, int C, uint64_t seed, uint32_t s
         require(w_dev && edges_dev && count_dev && offset_dev && src_dev && C > 0, "bad argument");
         hip_check(launch_cov_resample_src(w_dev, C, seed, stage, edges_dev, count_dev, offset_dev, src_dev, S(stream)),
                   "resample");
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free resampling
+// The scratch of altar_cov_resample_dev (the C + 1 bin edges and the C count
+// offsets) in a fixed device array, for callers that cannot allocate.  Every
+// device the library runs on holds its own copy for the life of the process.
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): it passes the address that
+// hipGetSymbolAddress returns for the current device.
+namespace {
+
+constexpr size_t NOALLOC_EDGE_BYTES = ((size_t)ALTAR_NOALLOC_MAX_CHAINS + 1) * sizeof(double);
+constexpr size_t NOALLOC_EDGE_SPAN = (NOALLOC_EDGE_BYTES + 255) / 256 * 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_EDGE_SPAN + (size_t)ALTAR_NOALLOC_MAX_CHAINS * sizeof(int);
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the current device's address of scratch (looked up once per device)
+void *scratch_address()
+{
+    int dev = -1;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    require(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device ordinal beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        require(ptr != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int altar_cov_resample_dev_noalloc(const double *w_dev, int C, uint64_t seed, uint32_t stage, int *count_dev,
+                                   int *src_dev, void *stream)
+{
+    return guarded([&] {
+        require(w_dev && count_dev && src_dev && C > 0, "bad argument");
+        require(C <= ALTAR_NOALLOC_MAX_CHAINS,
+                "resample (allocation-free): C = " + std::to_string(C) + " exceeds the scratch capacity of " +
+                    std::to_string(ALTAR_NOALLOC_MAX_CHAINS) + " chains");
+        unsigned char *base = (unsigned char *)scratch_address();
+        double *edges = (double *)base;
+        int *offset = (int *)(base + NOALLOC_EDGE_SPAN);
+        // every scratch word the kernels read is written first in this call
+        // (cumsum writes the C + 1 edges, the count scan the C offsets)
+        hip_check(launch_cov_resample_src(w_dev, C, seed, stage, edges, count_dev, offset, src_dev, S(stream)),
+                  "resample (allocation-free)");
     });
 }
 

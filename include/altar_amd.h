@@ -226,6 +226,17 @@ int altar_cov_update_dev(const double *llk_dev, int C, double beta, double toler
  * doubles) and offset_dev (C ints) are scratch */
 int altar_cov_resample_dev(const double *w_dev, int C, uint64_t seed, uint32_t stage, double *edges_dev,
                            int *count_dev, int *offset_dev, int *src_dev, void *stream);
+/* As altar_cov_resample_dev, with the scratch (bin edges and count offsets)
+ * in a fixed device array of the library instead of caller buffers.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  It calls no allocation routine.  C <= ALTAR_NOALLOC_MAX_CHAINS
+ * (larger C: ALTAR_EINVAL); the array holds 3 MiB per device for the life of
+ * the process.  All calls on one device share the array: calls that overlap
+ * on one device (different streams or host threads) race. */
+#define ALTAR_NOALLOC_MAX_CHAINS 262144
+int altar_cov_resample_dev_noalloc(const double *w_dev, int C, uint64_t seed, uint32_t stage, int *count_dev,
+                                   int *src_dev, void *stream);
 /* resampled state: row c <- old row src_dev[c] (theta, prior, data, posterior) */
 int altar_resample_gather(const double *theta_old, const double *prior_old, const double *data_old,
                           const double *post_old, const int *src_dev, int C, int P, double *theta,
