@@ -263,7 +263,8 @@ def test_joint_full_size_llk_vs_oracle(cuda, oracle, joint_full):
     model = Joint(case["psets"], case["datasets"], job=Job(chains=C)).initialize()
     assert model.data_count == 40600
     t = torch.from_numpy(theta).to(cuda)
-    for prec, tol in (("fp32", 2e-5), ("bf16r", 2.5e-4)):
+    # measured: fp32 1.2e-6, bf16r 4.2e-6
+    for prec, tol in (("fp32", 5e-6), ("bf16r", 1.5e-5)):
         llk = torch.empty(C, dtype=torch.float64, device=cuda)
         model.source.data_llk(t, llk, prec)
         got = llk.cpu().numpy()
@@ -302,4 +303,4 @@ def test_joint_full_size_65536_chains(cuda, oracle, joint_full):
     ref = joint_oracle_llk(case, fwd, theta[pick])
     rel = np.abs(la[pick] - ref) / np.abs(ref)
     print("joint 65,536 chains: llk range", la.min(), la.max(), "| slice vs oracle max rel", rel.max())
-    assert rel.max() < 2e-5
+    assert rel.max() < 2.5e-8      # measured 5.2e-9 (chains near the truth)
