@@ -111,7 +111,8 @@ class DelayedAcceptance(Metropolis):
         llk = torch.empty(n, dtype=torch.float64, device=theta.device)
         wr = torch.empty((n, sur.Npad), dtype=torch.float32, device=theta.device)
         _lib.call("altar_source_data_llk_fields", self.model_source(annealer).handle, _lib.ptr(theta), n,
-                  _lib.ptr(llk), _lib.PRECISIONS[annealer.model.precision], _lib.ptr(wr), sur.Npad, _lib.stream())
+                  _lib.ptr(llk), _lib.PRECISIONS[sur.fields_precision(annealer.model.precision)], _lib.ptr(wr),
+                  sur.Npad, _lib.stream())
         rmse = sur.field_rmse_device(theta, wr)
         sur.validation_rmse = rmse
         return rmse <= self.max_rmse

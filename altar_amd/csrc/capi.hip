@@ -705,12 +705,11 @@ int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C
                                  float *wr_dev, size_t ldw, void *stream)
 {
     return guarded([&] {
+        require(h != nullptr, "null handle");
         require(wr_dev != nullptr && ldw >= h->Npad, "fields: output row pitch must be >= the padded N");
-        require((h->wkind == ALTAR_WHITEN_LOWER || h->wkind == ALTAR_WHITEN_BLOCKS) &&
-                    !(h->model == ALTAR_LINEAR && h->prewhitened),
-                "fields: a lower-triangular or block whitener is required (linear sources in direct mode)");
+        require(h->wkind != 0 || (h->model == ALTAR_LINEAR && h->prewhitened), "fields: whitening operator not set");
         require(precision == ALTAR_FP32 || precision == ALTAR_BF16 || precision == ALTAR_BF16_REFINED,
-                "fields: precision must be FP32 or BF16");
+                "fields: precision must be FP32 or BF16 (the stored fields are FP32)");
         data_llk_impl(h, theta_dev, C, llk_dev, precision, wr_dev, ldw, S(stream));
     });
 }
@@ -718,6 +717,30 @@ int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C
 }  // extern "C"
 
 namespace {
+// Whitened fields of a diagonal whitener (or of a pre-whitened linear
+// residual, w = NULL): block (c, t) writes z = w r for the 128 columns
+// [128 t, 128 t + 128) of chain c (zero past N) and the sum of z^2 into
+// partial[c ntiles + t], the layout llk_reduce_kernel sums in tile order.
+__global__ __launch_bounds__(128) void diag_fields_kernel(const float *__restrict__ R, size_t ldr,
+                                                          const float *__restrict__ w, int N,
+                                                          float *__restrict__ out, size_t ldo,
+                                                          float *__restrict__ partial, int ntiles)
+{
+    const int c = blockIdx.x, t = blockIdx.y;
+    const int n = t * 128 + threadIdx.x;
+    float z = 0.0f;
+    if (n < N) {
+        const float r = R[(size_t)c * ldr + n];
+        z = w ? w[n] * r : r;
+    }
+    out[(size_t)c * ldo + n] = z;
+    __shared__ float half[2];
+    const float q = wave_sum(z * z);
+    if ((threadIdx.x & 63) == 0) half[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)c * ntiles + t] = half[0] + half[1];
+}
+
 void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision, float *wr_out,
                    size_t ldw, hipStream_t stream)
 {
@@ -733,6 +756,31 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         const size_t Np = h->Npad, Cp = h->Cpad, Mp = round_up((size_t)C, 128);
         const int ntn = (int)(Np / 128);
 
+        if (wr_out && (h->wkind == ALTAR_WHITEN_DIAG || (h->model == ALTAR_LINEAR && h->prewhitened))) {
+            // fields of a diagonal whitener or of a pre-whitened linear
+            // source: FP32 residuals, then z = w r per element with its
+            // per-tile sums of squares (no GEMM); FP32 whatever the
+            // whitening precision asked for
+            require(h->model != ALTAR_JOINT, "joint fields need a lower or block whitener");
+            require(h->model != ALTAR_LINEAR || h->prewhitened,
+                    "linear fields with a diagonal whitener: pre-whiten G and d instead");
+            h->R32.reserve(Cp * Np * sizeof(float), true);
+            if (h->model == ALTAR_LINEAR) {
+                linear_residual_f32(h, theta_dev, C, false, st);
+            } else {
+                Out o{};
+                o.ld = Np;
+                o.r32 = h->R32.as<float>();
+                run_source(h, theta_dev, C, MODE_RES_F32, false, false, o, st);
+            }
+            h->partial.reserve(Cp * ntn * sizeof(float));
+            const float *w = h->model == ALTAR_LINEAR ? nullptr : h->wd.as<float>();
+            hipLaunchKernelGGL(diag_fields_kernel, dim3((unsigned)C, (unsigned)ntn), dim3(128), 0, st,
+                               h->R32.as<float>(), Np, w, h->N, wr_out, ldw, h->partial.as<float>(), ntn);
+            hip_check(hipGetLastError(), "diagonal fields");
+            hip_check(launch_llk_reduce(h->partial.p, 0, ntn, C, h->normalization, llk_dev, st), "reduce");
+            return;
+        }
         if (h->model == ALTAR_LINEAR) {
             require(precision != ALTAR_FP64, "linear llk: FP64 is not offered");
             if (h->prewhitened) {

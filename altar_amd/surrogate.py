@@ -194,18 +194,29 @@ class Surrogate:
 
     def collect_llk(self, theta, llk_out, precision="fp32"):
         """Exact data llk of ``theta`` (C rows, C % 128 == 0) that also appends the
-        whitened residuals to the training set.  Returns the number stored."""
+        whitened residuals to the training set.  Returns the number stored.
+        The fields are FP32: a model that computes in FP64 gets them from an
+        FP32 evaluation beside its own FP64 llk."""
         C = theta.shape[0]
         if self.space() < C:
             self.model.source.data_llk(theta, llk_out, precision)
             return 0
         theta = theta.contiguous()
         dst = self.wr[self.count:self.count + C]
-        _lib.call("altar_source_data_llk_fields", self.model.source.handle, _lib.ptr(theta), C, _lib.ptr(llk_out),
-                  _lib.PRECISIONS[precision], _lib.ptr(dst), self.Npad, _lib.stream())
+        fields_llk = llk_out if precision != "fp64" else torch.empty_like(llk_out)
+        _lib.call("altar_source_data_llk_fields", self.model.source.handle, _lib.ptr(theta), C,
+                  _lib.ptr(fields_llk), _lib.PRECISIONS[self.fields_precision(precision)], _lib.ptr(dst), self.Npad,
+                  _lib.stream())
+        if precision == "fp64":
+            self.model.source.data_llk(theta, llk_out, precision)
         self.theta[self.count:self.count + C].copy_(theta)
         self.count += C
         return C
+
+    @staticmethod
+    def fields_precision(precision):
+        """The precision of a fields evaluation (FP32 fields for FP64 models)."""
+        return "fp32" if precision == "fp64" else precision
 
     # -- fitting
     def fit(self, epochs=40, batch=4096, lr=2e-3, holdout=0.05, pca_sample=40_000, lr_final=1e-4):

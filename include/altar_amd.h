@@ -132,9 +132,10 @@ int altar_source_data_llk(altar_source *h, const double *theta_dev, int C, doubl
 /* As altar_source_data_llk, and also stores the whitened residuals W r
  * (C x ldw FP32, ldw >= N rounded up to 128, C a multiple of 128): the
  * training fields of the surrogate, produced by the same evaluation.
- * Requires a lower-triangular whitener (linear sources: not pre-whitened) or
- * a block whitener whose blocks' padded ranges tile [0, N rounded up to 128)
- * (block b's columns hold W_b r_b, its padding columns zero). */
+ * Any whitener: diagonal (w r per element, FP32), lower triangular, the
+ * pre-whitened residual of a linear source, or a block whitener whose blocks'
+ * padded ranges tile [0, N rounded up to 128) (block b's columns hold
+ * W_b r_b, its padding columns zero).  Precision FP32 or BF16 (not FP64). */
 int altar_source_data_llk_fields(altar_source *h, const double *theta_dev, int C, double *llk_dev, int precision,
                                  float *wr_dev, size_t ldw, void *stream);
 

@@ -23,3 +23,30 @@ def test_traffic_summary_is_read():
     t = bench.measured_traffic("cdm_forward")
     assert t is None or t > 0.3e9    # at least the residual matrix (16,384 x 5,120 FP32) is written
     assert bench.CDM_FLOPS_PER_PAIR == 12 * 11 + 12 * 194 + 10
+
+
+def test_autocorrelation_times_on_ar1_chains():
+    """bench.autocorrelation_times on AR(1) chains x_t = phi x_{t-1} + e_t,
+    whose integrated autocorrelation time is (1 + phi) / (1 - phi): 1 for
+    independent draws, 19 at phi = 0.9, also with a drift common to all
+    chains added (the per-step centring removes it).  A record too short for
+    the sequence to end is flagged as a lower bound."""
+    import numpy as np
+    import torch
+    import bench
+    rng = np.random.default_rng(5)
+    L, C = 2000, 512
+    phis = np.array([0.0, 0.5, 0.9])
+    x = np.zeros((L, C, len(phis)))
+    x[0] = rng.standard_normal((C, len(phis))) / np.sqrt(1 - phis ** 2)
+    for t in range(1, L):
+        x[t] = phis * x[t - 1] + rng.standard_normal((C, len(phis)))
+    tau, cut, _ = bench.autocorrelation_times(torch.from_numpy(x).float())
+    want = (1 + phis) / (1 - phis)
+    assert np.all(np.abs(tau / want - 1) < 0.1), (tau, want)
+    assert not cut.any()
+    drift = np.linspace(0.0, 3.0, L)[:, None, None]
+    tau_d, cut_d, _ = bench.autocorrelation_times(torch.from_numpy(x + drift).float())
+    assert np.all(np.abs(tau_d / want - 1) < 0.1) and not cut_d.any()
+    tau_short, cut_short, _ = bench.autocorrelation_times(torch.from_numpy(x[:40]).float(), max_lag=5)
+    assert cut_short[2] and tau_short[2] < want[2]

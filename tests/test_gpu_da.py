@@ -317,3 +317,28 @@ def test_substep_scale_tuning(cuda):
     assert rates[0.2][0] > rates[0.5][0]
     for target, (_, rate) in rates.items():
         assert abs(rate - target) < 0.1
+
+
+def test_da_on_fp64_mogi_with_diagonal_cd(cuda):
+    """Delayed acceptance on the FP64 Mogi model with its diagonal Cd (the
+    1,000-point case): the fields are collected from FP32 evaluations beside
+    the chains' FP64 llk, the surrogate trains and validates, and the anneal
+    reaches beta = 1 with delayed-acceptance stages."""
+    from altar_amd.cases import mogi_case
+    from altar_amd.mogi import Mogi, default_psets as mogi_psets
+    print("case: DA on the FP64 Mogi model, diagonal Cd, C=1024 N=1000", flush=True)
+    case = mogi_case()
+    C, steps = 1024, 5
+    job = Job(chains=C, steps=steps, seed=17, tolerance=1e-3)
+    model = Mogi(mogi_psets(), case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
+                 precision="fp64", job=job).initialize()
+    sur = Surrogate(model, K=32, capacity=C * steps * 4, max_batch=C, seed=1)
+    da = DelayedAcceptance(sur, train_beta=0.05, max_rmse=1e9, steps=steps,
+                           fit_kwargs=dict(epochs=30, batch=1024, lr=2e-3))
+    ann = Annealer(da, COV(), seed=17).initialize(job)
+    ann.posterior(model)
+    modes = [m["mode"] for m in da.mode_history]
+    print("modes", modes, "validation rmse", sur.validation_rmse, "stages", len(ann.history))
+    assert sur.trained and "da" in modes and ann.worker.beta >= 1.0 - job.tolerance
+    for h in ann.history:
+        assert h["accepted"] + h["rejected"] + h["unlikely"] == C * steps

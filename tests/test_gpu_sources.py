@@ -317,3 +317,52 @@ def test_cdm_fp64_side_policy(cuda, oracle):
     assert errs[0.4] < 2e-5     # measured 3.5e-6
     with pytest.raises(RuntimeError):
         src.fp64_sides(-1.0)
+
+
+def test_fields_diagonal_and_prewhitened(cuda, oracle):
+    """altar_source_data_llk_fields with a diagonal whitener (Mogi) and with a
+    pre-whitened linear source: the fields equal w r and W (G theta - d) of the
+    oracle (zero in the padding columns), and the llk equals the plain entry
+    point's to FP32 rounding of the norm."""
+    from altar_amd import _lib as L
+    from altar_amd import Job
+    from altar_amd.linear import Linear, default_psets
+    rng = np.random.default_rng(45)
+    case = mogi_case(N=300)
+    src = make_mogi(case, 5)
+    norm = norms.normalization(case["cd"])
+    src.whitening("diag", 1.0 / np.sqrt(case["cd"]), norm)
+    C, Npad = 256, 384
+    theta = mogi_theta(rng, C)
+    t = torch.from_numpy(theta).to(cuda)
+    buf = torch.full((C, Npad), 7.0, dtype=torch.float32, device=cuda)
+    llk = torch.empty(C, dtype=torch.float64, device=cuda)
+    L.call("altar_source_data_llk_fields", src.handle, L.ptr(t), C, L.ptr(llk), L.FP32, L.ptr(buf), Npad, None)
+    res = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"], case["oid"], 0.25)
+    want = res / np.sqrt(case["cd"])[None, :]
+    b = buf.cpu().numpy()
+    err = np.max(np.abs(b[:, :300] - want)) / np.max(np.abs(want))
+    print("mogi diagonal fields: max error of the field max", err)
+    assert err < 1e-5 and np.all(b[:, 300:] == 0.0)
+    ref = torch.empty(C, dtype=torch.float64, device=cuda)
+    src.data_llk(t, ref, "fp32")
+    a, r = llk.cpu().numpy(), ref.cpu().numpy()
+    assert np.all(np.abs(a - r) <= 1e-6 * (abs(norm) + np.abs(r)))
+    # pre-whitened linear source
+    N, P = 300, 20
+    G = rng.normal(size=(N, P)) * 1e-2
+    cd = norms.exponential_covariance(rng.uniform(0, 3e4, (N, 2)), 25e-6, 5e3)
+    d = G @ rng.uniform(0, 2, P) + np.linalg.cholesky(cd) @ rng.normal(size=N)
+    m = Linear(default_psets(P), G, d, cd, job=Job(chains=C)).initialize()
+    th = rng.normal(1.0, 0.5, (C, P))
+    t = torch.from_numpy(th).to(cuda)
+    buf.fill_(7.0)
+    L.call("altar_source_data_llk_fields", m.source.handle, L.ptr(t), C, L.ptr(llk), L.FP32, L.ptr(buf), Npad, None)
+    want = oracle.linear_residuals(th, G, d) @ oracle.exact_whitener(cd).T
+    b = buf.cpu().numpy()
+    err = np.max(np.abs(b[:, :N] - want)) / np.max(np.abs(want))
+    print("pre-whitened linear fields: max error of the field max", err)
+    assert err < 1e-5 and np.all(b[:, N:] == 0.0)
+    m.source.data_llk(t, ref, "fp32")
+    a, r = llk.cpu().numpy(), ref.cpu().numpy()
+    assert np.all(np.abs(a - r) <= 1e-6 * (abs(m.normalization) + np.abs(r)))
