@@ -69,6 +69,7 @@ SIGNATURES = {
                              _vp, _vp, _vp, _vp]),
     "altar_mh_scaling": (_d, [_d, _d, _ull, _ull, _ull]),
     "altar_cov_dbeta_grid": (_i, [_vp, _i, _d, _d, _d, _d, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "altar_cov_dbeta_brent": (_i, [_vp, _i, _d, _d, _d, _d, _i, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "altar_cov_covariance": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "altar_cov_multiplicities": (_i, [_vp, _i, _u64, _u32, _vp]),
     "altar_cov_update_dev": (_i, [_vp, _i, _d, _d, _d, _vp, _vp, _vp]),

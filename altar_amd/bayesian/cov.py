@@ -10,6 +10,10 @@ Two placements of the stage:
   (libaltar's dbeta_grid, COV.cc:222-368) on a copy of the data
   log-likelihoods, bit for bit as the oracle; covariance and gather on the GPU.
 
+Solvers: ``Grid`` (the reference's default, Solver.py:45-53) and ``Brent``
+(Brent.py, libaltar dbeta_brent, COV.cc:58-219; host FP64 in either
+placement, its weights then feed the device stage).
+
 Resampled rows are laid out in the order of the original samples (each
 sample repeated by its multiplicity) rather than by decreasing multiplicity
 (COV.py:159-208); the resampled set is the same multiset.
@@ -41,6 +45,29 @@ class Grid:
         b, c = ctypes.c_double(), ctypes.c_double()
         _lib.call("altar_cov_dbeta_grid", _lib.ptr(llk), len(llk), median, float(beta), self.tolerance,
                   self.target, _lib.ptr(w), ctypes.byref(b), ctypes.byref(c))
+        return b.value, c.value, w
+
+
+class Brent:
+    """dbeta solver by Brent's minimization of (COV - target)^2 (Brent.py:19-63,
+    libaltar dbeta_brent, COV.cc:58-219), FP64 on the host."""
+
+    def __init__(self, tolerance=0.01, maxiter=1000):
+        self.tolerance = tolerance
+        self.maxiter = maxiter
+
+    def initialize(self, application=None, scheduler=None):
+        self.target = scheduler.target if scheduler is not None else 1.0
+        return self
+
+    def solve(self, llk, beta):
+        """llk: host FP64 array.  Returns (new beta, cov, normalized weights)."""
+        llk = np.ascontiguousarray(llk, dtype=np.float64)
+        median = float(np.median(llk))
+        w = np.empty_like(llk)
+        b, c = ctypes.c_double(), ctypes.c_double()
+        _lib.call("altar_cov_dbeta_brent", _lib.ptr(llk), len(llk), median, float(beta), self.tolerance,
+                  self.target, int(self.maxiter), _lib.ptr(w), ctypes.byref(b), ctypes.byref(c))
         return b.value, c.value, w
 
 
@@ -82,7 +109,10 @@ class COV:
         return ws
 
     def updateTemperature(self, step):
-        if self.resident:
+        # the device stage implements the grid solver; another solver runs on
+        # the host, and its weights go back to the device for the covariance
+        # and the resampling
+        if self.resident and isinstance(self.solver, Grid):
             ws = self._workspace(step.samples, step.data.device)
             _lib.call("altar_cov_update_dev", _lib.ptr(step.data), step.samples, float(step.beta),
                       self.solver.tolerance, self.target, _lib.ptr(ws["w"]), _lib.ptr(ws["out"]), _lib.stream())
@@ -94,6 +124,9 @@ class COV:
         llk = step.data.cpu().numpy()
         beta, self.cov, self.w = self.solver.solve(llk, step.beta)
         self.w_dev = torch.from_numpy(self.w).to(step.data.device)
+        if self.resident:
+            self._workspace(step.samples, step.data.device)["w"].copy_(self.w_dev)
+            self.w_dev = self._ws["w"]
         return beta
 
     def computeCovariance(self, step):

@@ -50,7 +50,9 @@ def build(verbose=False, jobs=4):
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + \
+        ["-L", os.path.join(rocm, "lib"), "-lrocblas", "-Wl,-rpath," + os.path.join(rocm, "lib")]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -128,7 +129,10 @@ struct altar_source {
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
-    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census, fp64_list, fp64_count;
+    DevBuf R32, Rhi, Rlo, R64, Y64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census, fp64_list,
+        fp64_count;
+    // rocBLAS handle of the FP64 whitening (dtrmm), created on first use
+    rocblas_handle blas = nullptr;
     // optional per-kernel timing with HIP events on the launch stream
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[ALTAR_TIMER_SLOTS];
@@ -155,7 +159,8 @@ struct altar_source {
         Ccap = C;
         Cpad = round_up((size_t)C, 128);
         // zero-initialised: the GEMMs read the padding columns
-        R32.release(); Rhi.release(); Rlo.release(); R64.release(); T32.release(); Thi.release(); Tlo.release();
+        R32.release(); Rhi.release(); Rlo.release(); R64.release(); Y64.release(); T32.release(); Thi.release();
+        Tlo.release();
         partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release(); census.release();
         fp64_list.release(); fp64_count.release();
     }
@@ -210,6 +215,8 @@ int altar_source_destroy(altar_source *h)
         h->census.release();
         h->fp64_list.release();
         h->fp64_count.release();
+        h->Y64.release();
+        if (h->blas) (void)rocblas_destroy_handle(h->blas);
         for (WBlock &b : h->blocks) {
             b.W32.release(); b.Whi.release(); b.Wlo.release();
         }
@@ -827,15 +834,36 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         // GEMM per block over the block's columns of R)
         const bool blocked = h->wkind == ALTAR_WHITEN_BLOCKS;
         if (precision == ALTAR_FP64) {
-            require(h->model == ALTAR_MOGI && !blocked, "FP64 full whitening is offered for the Mogi source only");
-            h->R64.reserve((size_t)C * h->N * sizeof(double));
+            // the reference's precision: FP64 residuals (Mogi in FP64; CDM
+            // with every side in FP64), W r by rocBLAS dtrmm, FP64 sums
+            require(!blocked && (h->model == ALTAR_MOGI || h->model == ALTAR_CDM),
+                    "FP64 full whitening is offered for Mogi and CDM sources with a lower-triangular whitener");
+            require(h->model != ALTAR_CDM || h->fp64_sin > 1.0,
+                    "CDM FP64 llk: evaluate every side in FP64 first (altar_source_fp64_sides(h, 2))");
+            const size_t N = (size_t)h->N;
+            h->R64.reserve(Cp * N * sizeof(double));
+            h->Y64.reserve(Cp * N * sizeof(double));
             Out o{};
-            o.ld = h->N;
+            o.ld = N;
             o.r64 = h->R64.as<double>();
-            run_source(h, theta_dev, C, MODE_RES_F64, true, false, o, st);
-            hip_check(launch_whiten_f64(h->R64.as<double>(), h->N, h->W64.as<double>(), h->N, h->N, C,
-                                        h->normalization, llk_dev, st),
-                      "whiten f64");
+            run_source(h, theta_dev, C, MODE_RES_F64, h->model == ALTAR_MOGI, false, o, st);
+            if (!h->blas) {
+                rocblas_handle b = nullptr;
+                require(rocblas_create_handle(&b) == rocblas_status_success, "rocblas_create_handle failed");
+                h->blas = b;
+            }
+            require(rocblas_set_stream(h->blas, st) == rocblas_status_success, "rocblas_set_stream failed");
+            // row-major R (C x N) is column-major X = R^T (N x C); row-major
+            // lower W is column-major upper W^T; Y^T = W X = op(W^T) X with op = T
+            const double one = 1.0;
+            Mark m(h, 1, st);
+            require(rocblas_dtrmm(h->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                  rocblas_diagonal_non_unit, (rocblas_int)N, (rocblas_int)C, &one,
+                                  h->W64.as<double>(), (rocblas_int)N, h->R64.as<double>(), (rocblas_int)N,
+                                  h->Y64.as<double>(), (rocblas_int)N) == rocblas_status_success,
+                    "rocblas_dtrmm failed");
+            hip_check(launch_sumsq_rows_f64(h->Y64.as<double>(), N, (int)N, C, h->normalization, llk_dev, st),
+                      "FP64 sums of squares");
             return;
         }
         int ntiles = ntn;
@@ -1141,6 +1169,80 @@ int altar_cov_dbeta_grid(const double *llk, int C, double median, double beta, d
         std::memcpy(w_out, w.data(), sizeof(double) * C);
         *beta_out = beta;
         *cov_out = cov;
+    });
+}
+
+int altar_cov_dbeta_brent(const double *llk, int C, double median, double beta, double tolerance, double target,
+                          int maxiter, double *w_out, double *beta_out, double *cov_out)
+{
+    return guarded([&] {
+        require(llk && w_out && beta_out && cov_out && C > 1 && maxiter >= 0, "bad argument");
+        require(beta >= 0.0 && beta <= 1.0, "beta outside [0, 1]");
+        std::vector<double> w(C);
+        double last_cov = 0.0;
+        // the minimized metric (COV - target)^2, 1e100 where the COV is not finite (COV.cc:355-362)
+        auto metric = [&](double dbeta) {
+            last_cov = cov_of(dbeta, llk, C, median, w);
+            return last_cov == 1e100 ? 1e100 : (last_cov - target) * (last_cov - target);
+        };
+        double lo = 0.0, hi = 1.0 - beta;
+        metric(hi);
+        if (last_cov < target || std::fabs(last_cov - target) < tolerance) {   // skip to beta = 1
+            std::memcpy(w_out, w.data(), sizeof(double) * C);
+            *beta_out = 1.0;
+            *cov_out = last_cov;
+            return;
+        }
+        // Brent's method (Brent 1973, localmin): golden-section steps, parabolic
+        // steps through the best three points when they fall inside the bracket
+        const double golden = 0.5 * (3.0 - std::sqrt(5.0));
+        const double rel_eps = 1.4901161193847656e-08, abs_eps = 1e-12;
+        double best = 5.0e-5, f_best = metric(best);
+        double second = best, third = best, f_second = f_best, f_third = f_best;
+        double step = 0.0, prev_step = 0.0;
+        for (int it = 0; it < maxiter && !(std::fabs(last_cov - target) < tolerance); ++it) {
+            const double mid = 0.5 * (lo + hi);
+            const double t1 = rel_eps * std::fabs(best) + abs_eps, t2 = 2.0 * t1;
+            if (std::fabs(best - mid) <= t2 - 0.5 * (hi - lo)) break;
+            double num = 0.0, den = 0.0, older = 0.0;
+            if (std::fabs(prev_step) > t1) {
+                const double r = (best - second) * (f_best - f_third);
+                den = (best - third) * (f_best - f_second);
+                num = (best - third) * den - (best - second) * r;
+                den = 2.0 * (den - r);
+                if (den > 0.0) num = -num; else den = -den;
+                older = prev_step;
+                prev_step = step;
+            }
+            if (std::fabs(num) < std::fabs(0.5 * den * older) && num > den * (lo - best) && num < den * (hi - best)) {
+                step = num / den;
+                const double trial = best + step;
+                if (trial - lo < t2 || hi - trial < t2) step = best < mid ? t1 : -t1;
+            } else {
+                prev_step = best < mid ? hi - best : lo - best;
+                step = golden * prev_step;
+            }
+            const double u = std::fabs(step) >= t1 ? best + step : best + (step > 0.0 ? t1 : -t1);
+            const double fu = metric(u);
+            if (fu <= f_best) {
+                if (u < best) hi = best; else lo = best;
+                third = second; f_third = f_second;
+                second = best; f_second = f_best;
+                best = u; f_best = fu;
+            } else {
+                if (u < best) lo = u; else hi = u;
+                if (fu <= f_second || second == best) {
+                    third = second; f_third = f_second;
+                    second = u; f_second = fu;
+                } else if (fu <= f_third || third == best || third == second) {
+                    third = u; f_third = fu;
+                }
+            }
+        }
+        metric(best);
+        std::memcpy(w_out, w.data(), sizeof(double) * C);
+        *beta_out = beta + best;
+        *cov_out = last_cov;
     });
 }
 

@@ -345,29 +345,21 @@ __global__ void llk_reduce_kernel(const T *__restrict__ partial, int ntiles, int
     llk[c] = normalization - 0.5 * s;
 }
 
-// FP64 reference-precision whitening for small problems: one block per chain,
-// |W r|^2 with W lower triangular.
-__global__ void whiten_f64_kernel(const double *__restrict__ R, size_t ldr, const double *__restrict__ W,
-                                  size_t ldw, int N, int C, double normalization, double *__restrict__ llk)
+// llk[c] = normalization - |y_c|^2 / 2 for the FP64 rows y_c (N values, row
+// pitch ld): one block per chain, each thread a strided partial sum, the
+// wave sums then the four waves in order (deterministic).
+__global__ __launch_bounds__(256) void sumsq_rows_f64_kernel(const double *__restrict__ Y, size_t ld, int N,
+                                                             double normalization, double *__restrict__ llk)
 {
     const int c = blockIdx.x;
-    const double *r = R + (size_t)c * ldr;
+    const double *y = Y + (size_t)c * ld;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        const double *w = W + (size_t)i * ldw;
-        double v = 0.0;
-        for (int j = 0; j <= i; ++j) v += w[j] * r[j];
-        acc += v * v;
-    }
-    __shared__ double red[256 / 64];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) acc += y[i] * y[i];
+    __shared__ double red[4];
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (int)blockDim.x / 64; ++w) s += red[w];
-        llk[c] = normalization - 0.5 * s;
-    }
+    if (threadIdx.x == 0) llk[c] = normalization - 0.5 * (((red[0] + red[1]) + red[2]) + red[3]);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -466,10 +458,10 @@ hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, d
     return hipGetLastError();
 }
 
-hipError_t launch_whiten_f64(const double *R, size_t ldr, const double *W, size_t ldw, int N, int C,
-                             double normalization, double *llk, hipStream_t st)
+hipError_t launch_sumsq_rows_f64(const double *Y, size_t ld, int N, int C, double normalization, double *llk,
+                                 hipStream_t st)
 {
-    hipLaunchKernelGGL(whiten_f64_kernel, dim3(C), dim3(256), 0, st, R, ldr, W, ldw, N, C, normalization, llk);
+    hipLaunchKernelGGL(sumsq_rows_f64_kernel, dim3(C), dim3(256), 0, st, Y, ld, N, normalization, llk);
     return hipGetLastError();
 }
 

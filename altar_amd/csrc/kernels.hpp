@@ -118,8 +118,9 @@ hipError_t launch_transpose_f32(const float *src, size_t lds, int rows, int cols
                                 hipStream_t st);
 hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
                              double *llk, hipStream_t st);
-hipError_t launch_whiten_f64(const double *R, size_t ldr, const double *W, size_t ldw, int N, int C,
-                             double normalization, double *llk, hipStream_t st);
+// llk[c] = normalization - |Y[c, 0:N]|^2 / 2 in FP64 (one block per chain)
+hipError_t launch_sumsq_rows_f64(const double *Y, size_t ld, int N, int C, double normalization, double *llk,
+                                 hipStream_t st);
 
 // sampler.hip
 hipError_t launch_normals(double *xi, int C, int P, uint64_t seed, uint32_t stage, uint32_t step, uint32_t tag,

@@ -178,9 +178,11 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
     const cdm::Geom<float> &G = g32[chain];
     const unsigned fl = flags[chain];
     float val = 0.0f;
+    double val64 = 0.0;   // MODE_RES_F64: the residual formed in FP64
     if (n < obs.N) {
         if (fl & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) {
             val = __builtin_nanf("");
+            val64 = __builtin_nan("");
         } else {
             const float X = obs.x[n], Y = obs.y[n];
             const float k = 1.0f - 2.0f * nu;
@@ -240,7 +242,10 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                 float shift = 0.0f;
                 const int id = obs.oid[n];
                 for (int q = 0; q < nsets; ++q) shift = (id == q) ? sh[q] : shift;
-                val = u - shift - obs.d[n];
+                if constexpr (MODE == MODE_RES_F64)
+                    val64 = (double)u - (double)shift - obs.dd[n];
+                else
+                    val = u - shift - obs.d[n];
             }
         }
     }
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
     } else if constexpr (MODE == MODE_RES_F32) {
         if (n < obs.N) out.r32[(size_t)chain * out.ld + n] = val;
     } else if constexpr (MODE == MODE_RES_F64) {
-        if (n < obs.N) out.r64[(size_t)chain * out.ld + n] = (double)val;
+        if (n < obs.N) out.r64[(size_t)chain * out.ld + n] = val64;
     } else {
         if (n < obs.N) {
             __hip_bfloat16 hi = __float2bfloat16(val);

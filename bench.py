@@ -340,7 +340,8 @@ def main():
     variants = {}
     if not args.no_other:
         for label, prec, thr in (("bf16r", "bf16r", args.fp64_sides),
-                                 ("fp32_every_side_fp64", "fp32", 2.0)):
+                                 ("fp32_every_side_fp64", "fp32", 2.0),
+                                 ("fp64_reference_precision", "fp64", 2.0)):
             variants[label] = cdm_variant_timing(annealer, model, step, prec, thr, args.steps)
         model.precision = args.precision
         model.source.fp64_sides(args.fp64_sides)
@@ -391,6 +392,8 @@ def cdm_variant_timing(annealer, model, step, precision, fp64_sides, steps):
             "accepted_samples_per_s": acc / elapsed, "cdm_forward_ms": fwd,
             "cdm_forward_tflops": C * N * CDM_FLOPS_PER_PAIR / (fwd * 1e-3) / 1e12,
             "whitening_ms": gemm, "whitening_tflops_algorithmic": C * N * (N + 1) / (gemm * 1e-3) / 1e12,
+            "whitening": {"fp32": "gemm_nt_f32 (MFMA)", "bf16r": "gemm_nt_bf16x3 (MFMA)",
+                          "fp64": "rocBLAS dtrmm"}.get(precision),
             "fp64_sides_per_chain": f64s / max(C - invalid, 1)}
 
 

@@ -209,6 +209,14 @@ double altar_mh_scaling(double acceptance_weight, double rejection_weight, unsig
  * (C) and returns the new beta and the attained COV. */
 int altar_cov_dbeta_grid(const double *llk_host, int C, double median, double beta, double tolerance,
                          double target, double *w_host, double *beta_out, double *cov_out);
+/* dbeta by Brent's method on host FP64 data (libaltar dbeta_brent,
+ * COV.cc:58-219, Brent.py:19-63; the reference uses GSL's Brent minimizer):
+ * minimizes (COV - target)^2 over [0, 1 - beta] from 5e-5, stopping when the
+ * last COV is within tolerance of the target or after maxiter iterations
+ * (the reference's default 1000); the skip to beta = 1 as in the grid solver.
+ * Writes w (C) and returns the new beta and the attained COV. */
+int altar_cov_dbeta_brent(const double *llk_host, int C, double median, double beta, double tolerance,
+                          double target, int maxiter, double *w_host, double *beta_out, double *cov_out);
 /* weighted mean (P) and covariance Sigma (P x P) of theta_dev with weights w_dev */
 int altar_cov_covariance(const double *theta_dev, const double *w_dev, int C, int P, double *mean_dev,
                          double *sigma_dev, void *stream);

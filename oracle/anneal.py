@@ -1,7 +1,7 @@
 """CATMIP on the CPU with the oracle's FP64 pieces (test infrastructure only).
 
 Follows altar/altar/bayesian/Annealer.py:78-135 (stage loop),
-COV.py:65-251 (dbeta grid, weighted covariance, resampling) and
+COV.py:65-251 (dbeta grid or Brent, weighted covariance, resampling) and
 Metropolis.py:93-281 (propose, verify, accept, scaling), drawing from the
 same Philox streams as the GPU path: resampling (stage, 0, tag 2), proposal
 normals (stage, step, tag 0), acceptance dice (stage, step, tag 1), initial
@@ -13,7 +13,7 @@ from . import oracle as o
 
 
 def anneal(data_llk, kinds, lo, hi, prep_lo, prep_hi, C, steps, seed, tolerance=1e-3,
-           verify_model=None, scaling=0.1, max_stages=200):
+           verify_model=None, scaling=0.1, max_stages=200, solver="grid"):
     """Returns a dict with the final state and the per-stage history."""
     kinds = np.asarray(kinds, np.int32)
     P = len(kinds)
@@ -34,7 +34,7 @@ def anneal(data_llk, kinds, lo, hi, prep_lo, prep_hi, C, steps, seed, tolerance=
     stage = 0
     while beta + tolerance < 1 and stage < max_stages:
         stage += 1
-        new_beta, cov, w = o.dbeta_grid(data, beta)
+        new_beta, cov, w = o.dbeta_grid(data, beta) if solver == "grid" else o.dbeta_brent(data, beta)
         mean, sigma = o.weighted_covariance(theta, w)
         count = o.multiplicities(w, o.uniforms(seed, stage, 0, 2, C))
         src = np.repeat(np.arange(C), count)

@@ -568,6 +568,84 @@ double oracle_dbeta_grid(const double *llk, int C, double median, double *w,
     return dbeta;
 }
 
+/* COV.cc:58-219 (dbeta_brent): minimize f(dbeta) = (COV(dbeta) - target)^2
+ * over [0, 1 - beta] from the guess 5e-5, stopping when the COV of the last
+ * evaluated point is within tolerance of the target (gsl_root_test_residual)
+ * or after maxiter iterations; then dbeta = the best point.  The reference
+ * runs GSL's Brent minimizer (gsl_min_fminimizer_brent, GSL version not
+ * pinned); this is Brent's method as published (Brent 1973, "Algorithms for
+ * Minimization without Derivatives", ch. 5, localmin): golden-section steps
+ * with parabolic interpolation through the three best points.  The skip to
+ * beta = 1 is COV.cc's.                                                     */
+static double brent_f(double dbeta, const double *llk, int C, double median, double target, double *w,
+                      double *cov)
+{
+    return oracle_cov_eval(dbeta, llk, C, median, target, w, cov);
+}
+
+double oracle_dbeta_brent(const double *llk, int C, double median, double *w, double *beta_io,
+                          double tolerance, double target, int maxiter, double *cov_out)
+{
+    double a = 0.0, b = 1.0 - *beta_io;
+    double cov;
+    brent_f(b, llk, C, median, target, w, &cov);
+    if (cov < target || fabs(cov - target) < tolerance) {
+        *beta_io = 1.0;
+        *cov_out = cov;
+        return b;
+    }
+    const double gold = 0.5 * (3.0 - sqrt(5.0));
+    const double reps = 1.4901161193847656e-08, aeps = 1e-12;
+    double x = 5.0e-5, fx = brent_f(x, llk, C, median, target, w, &cov);
+    double v = x, w_ = x, fv = fx, fw = fx, d = 0.0, e = 0.0, last_cov = cov;
+    int iter = 0;
+    while (iter < maxiter && !(fabs(last_cov - target) < tolerance)) {
+        ++iter;
+        const double m = 0.5 * (a + b);
+        const double tol1 = reps * fabs(x) + aeps, tol2 = 2.0 * tol1;
+        if (fabs(x - m) <= tol2 - 0.5 * (b - a)) break;
+        double p = 0.0, q = 0.0, r = 0.0;
+        if (fabs(e) > tol1) {
+            r = (x - w_) * (fx - fv);
+            q = (x - v) * (fx - fw);
+            p = (x - v) * q - (x - w_) * r;
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p; else q = -q;
+            r = e;
+            e = d;
+        }
+        if (fabs(p) < fabs(0.5 * q * r) && p > q * (a - x) && p < q * (b - x)) {
+            d = p / q;
+            const double u = x + d;
+            if (u - a < tol2 || b - u < tol2) d = x < m ? tol1 : -tol1;
+        } else {
+            e = x < m ? b - x : a - x;
+            d = gold * e;
+        }
+        const double u = fabs(d) >= tol1 ? x + d : x + (d > 0.0 ? tol1 : -tol1);
+        const double fu = brent_f(u, llk, C, median, target, w, &cov);
+        last_cov = cov;
+        if (fu <= fx) {
+            if (u < x) b = x; else a = x;
+            v = w_; fv = fw;
+            w_ = x; fw = fx;
+            x = u; fx = fu;
+        } else {
+            if (u < x) a = u; else b = u;
+            if (fu <= fw || w_ == x) {
+                v = w_; fv = fw;
+                w_ = u; fw = fu;
+            } else if (fu <= fv || v == x || v == w_) {
+                v = u; fv = fu;
+            }
+        }
+    }
+    brent_f(x, llk, C, median, target, w, &cov);
+    *cov_out = cov;
+    *beta_io += x;
+    return x;
+}
+
 /* altar/altar/bayesian/COV.py:105-156: Sigma = sum w theta theta^T - mean mean^T */
 void oracle_weighted_covariance(const double *theta, const double *w, int C, int P,
                                 double *mean, double *sigma)

@@ -69,6 +69,7 @@ def _declare(lib):
                                                 ctypes.c_long, ctypes.c_long]),
         "oracle_cov_eval": (c_double, [c_double, _d, c_int, c_double, c_double, _d, _d]),
         "oracle_dbeta_grid": (c_double, [_d, c_int, c_double, _d, _d, c_double, c_double, _d]),
+        "oracle_dbeta_brent": (c_double, [_d, c_int, c_double, _d, _d, c_double, c_double, c_int, _d]),
         "oracle_weighted_covariance": (None, [_d, _d, c_int, c_int, _d, _d]),
         "oracle_multiplicities": (None, [_d, c_int, _d, _i]),
     }
@@ -295,6 +296,17 @@ def dbeta_grid(llk, beta, tolerance=0.01, target=1.0):
     cov = ctypes.c_double()
     lib().oracle_dbeta_grid(_p(llk), len(llk), median(llk), _p(w), ctypes.byref(b),
                             tolerance, target, ctypes.byref(cov))
+    return b.value, cov.value, w
+
+
+def dbeta_brent(llk, beta, tolerance=0.01, target=1.0, maxiter=1000):
+    """Returns (new beta, cov, w) per COV.cc:58-219 (Brent's method)."""
+    llk = _f64(llk)
+    w = np.zeros_like(llk)
+    b = ctypes.c_double(beta)
+    cov = ctypes.c_double()
+    lib().oracle_dbeta_brent(_p(llk), len(llk), median(llk), _p(w), ctypes.byref(b), tolerance, target, maxiter,
+                             ctypes.byref(cov))
     return b.value, cov.value, w
 
 

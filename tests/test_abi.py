@@ -68,3 +68,19 @@ def test_multiplicities_match_oracle(oracle):
 def test_scaling_matches_oracle(oracle):
     for a, r, u in ((10, 5, 85), (0, 0, 10), (100, 0, 0)):
         assert _lib.lib().altar_mh_scaling(8.0, 1.0, a, r, u) == oracle.mh_adjust_scaling(a, r, u)
+
+
+def test_dbeta_brent_matches_oracle_bit_for_bit(oracle):
+    """The host Brent solver against the oracle's restatement: same steps of
+    Brent's method, so the same dbeta, COV and weights bit for bit."""
+    import ctypes
+    rng = np.random.default_rng(61)
+    for C, scale, beta0 in ((3000, 40.0, 0.0), (1001, 3.0, 0.3), (4096, 0.2, 0.8), (500, 15.0, 0.95)):
+        llk = rng.normal(-700, scale, C)
+        med = float(np.median(llk))
+        w = np.empty(C)
+        b, c = ctypes.c_double(), ctypes.c_double()
+        _lib.call("altar_cov_dbeta_brent", _lib.ptr(llk), C, med, beta0, 0.01, 1.0, 1000, _lib.ptr(w),
+                  ctypes.byref(b), ctypes.byref(c))
+        rb, rc, rw = oracle.dbeta_brent(llk, beta0)
+        assert b.value == rb and c.value == rc and np.array_equal(w, rw), (C, scale, beta0)

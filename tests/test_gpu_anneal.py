@@ -8,7 +8,8 @@
   Both COV placements run: on the host (the bit-for-bit restatement of
   libaltar's dbeta_grid) and resident on the GPU (COV(resident=True), the
   default: median, grid search, weights, resampling and the weighted
-  covariance in device kernels).
+  covariance in device kernels); and the Brent solver (libaltar's
+  dbeta_brent) against the oracle's Brent.
 * Linear-Gaussian: the annealed posterior moments of 6 replicate anneals
   (independent seeds) agree with the closed form Sigma = (G^T Cd^-1 G + I)^-1,
   mu = Sigma G^T Cd^-1 d within 3 standard errors measured from the scatter
@@ -26,17 +27,20 @@ from replicates import assert_moments, replicate_moments, summary
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("C,N,steps,resident", [(512, 200, 3, False), (1024, 1000, 5, False),
-                                                (1024, 1000, 5, True)])
-def test_mogi_anneal_matches_oracle(cuda, oracle, C, N, steps, resident):
+@pytest.mark.parametrize("C,N,steps,resident,solver", [(512, 200, 3, False, "grid"), (1024, 1000, 5, False, "grid"),
+                                                       (1024, 1000, 5, True, "grid"),
+                                                       (1024, 1000, 5, True, "brent")])
+def test_mogi_anneal_matches_oracle(cuda, oracle, C, N, steps, resident, solver):
     from oracle.anneal import anneal
+    from altar_amd.bayesian import Brent, Grid
     from altar_amd.mogi import Mogi, default_psets
-    print(f"case: mogi anneal C={C} N={N} steps={steps} resident COV={resident}", flush=True)
+    print(f"case: mogi anneal C={C} N={N} steps={steps} resident COV={resident} solver={solver}", flush=True)
     case = mogi_case(N=N)
     job = Job(chains=C, steps=steps, seed=11, tolerance=1e-3)
     model = Mogi(default_psets(), case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"],
                  precision="fp64", job=job).initialize()
-    annealer = Annealer(Metropolis(steps=steps), COV(resident=resident), seed=11).initialize(job)
+    annealer = Annealer(Metropolis(steps=steps), COV(resident=resident, solver=Brent() if solver == "brent" else Grid()),
+                        seed=11).initialize(job)
     annealer.posterior(model)
     gpu = annealer.worker.step
     W = 1.0 / np.sqrt(case["cd"])
@@ -48,7 +52,7 @@ def test_mogi_anneal_matches_oracle(cuda, oracle, C, N, steps, resident):
         return oracle.l2_llk(r, W, norm)
 
     kinds, lo, hi = flatten(model.psets, "prior")
-    ref = anneal(llk, kinds, lo, hi, lo, hi, C, steps, 11)
+    ref = anneal(llk, kinds, lo, hi, lo, hi, C, steps, 11, solver=solver)
     gb = np.array([h["beta"] for h in annealer.history])
     rb = np.array([h["beta"] for h in ref["history"]])
     print("stages gpu", len(gb), "oracle", len(rb), "max beta difference",

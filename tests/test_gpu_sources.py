@@ -366,3 +366,41 @@ def test_fields_diagonal_and_prewhitened(cuda, oracle):
     m.source.data_llk(t, ref, "fp32")
     a, r = llk.cpu().numpy(), ref.cpu().numpy()
     assert np.all(np.abs(a - r) <= 1e-6 * (abs(m.normalization) + np.abs(r)))
+
+
+def test_cdm_llk_fp64_reference_precision(cuda, oracle):
+    """The reference's precision end to end: every CDM side in FP64 (FP64
+    atan2), the residual formed in FP64, W r by rocBLAS dtrmm in FP64 and FP64
+    sums of squares, against the oracle's FP64 llk.  With FP32 sides the FP64
+    llk is refused (the sides would not be reference precision)."""
+    print("case: cdm llk FP64 reference precision C=64 N=1200", flush=True)
+    rng = np.random.default_rng(16)
+    N, C = 1200, 64
+    x = rng.uniform(-15e3, 15e3, N)
+    y = rng.uniform(-15e3, 15e3, N)
+    los = unit_los(N)
+    truth = np.array([[1000.0, -500.0, 4000.0, 0.1, 1000.0, 1200.0, 800.0, 5.0, -8.0, 30.0]])
+    cd = norms.exponential_covariance(np.stack([x, y], 1), 64e-6, 5e3)
+    u = oracle.cdm_residuals(truth, (0, 2, 3, 4, 7, -1), x, y, los, np.zeros(N), np.zeros(N, np.int32), 0.25)[0]
+    d = u + np.linalg.cholesky(cd) @ rng.standard_normal(N)
+    theta = cdm_theta(rng, C, oracle)
+    theta[:16] = truth + rng.normal(0, [20, 20, 20, 1e-3, 10, 10, 10, 0.2, 0.2, 0.2], (16, 10))
+    res = oracle.cdm_residuals(theta, (0, 2, 3, 4, 7, -1), x, y, los, d, np.zeros(N, np.int32), 0.25)
+    norm = norms.normalization(cd)
+    kind, W = norms.whitener(cd)
+    ref = oracle.l2_llk(res, oracle.exact_whitener(cd), norm)
+    src = Source(_lib.CDM, 10, N, 0.25)
+    src.layout([0, 2, 3, 4, 7, -1, 0])
+    src.observations(x, y, los, d, np.zeros(N, np.int32))
+    src.whitening(kind, W, norm)
+    t = torch.from_numpy(theta).to(cuda)
+    out = torch.empty(C, dtype=torch.float64, device=cuda)
+    with pytest.raises(RuntimeError):
+        src.data_llk(t, out, "fp64")
+    src.fp64_sides(2.0)
+    src.data_llk(t, out, "fp64")
+    got = out.cpu().numpy()
+    rel = np.abs(got - ref) / np.abs(ref)
+    print("fp64 reference precision: max rel llk err", rel.max(), "abs max", np.abs(got - ref).max(),
+          "near truth", np.abs(got - ref)[:16].max())
+    assert rel.max() < 3e-11     # measured 6.3e-12

@@ -147,6 +147,36 @@ def test_dbeta_grid_against_scan(oracle):
     assert b1 == 1.0
 
 
+def test_dbeta_brent_pins(oracle):
+    """oracle_dbeta_brent (COV.cc:58-219; GSL's Brent minimizer in the
+    reference, whose exact iterates are not pinned here): Brent's method on
+    (COV - 1)^2 over [0, 1 - beta].
+    * Gaussian llk of sd s at C = 10^5: the COV of lognormal weights reaches 1
+      at dbeta = sqrt(ln 2) / s; the solver's COV is within its tolerance of
+      1 and its dbeta within the matching band of the closed form;
+    * it agrees with the grid solver, both within tolerance of the target;
+    * the skip to beta = 1 when the full step stays below the target."""
+    rng = np.random.default_rng(62)
+    for s in (40.0, 8.0, 2.0):
+        llk = rng.normal(-1000.0, s, 100_000)
+        b, c, w = oracle.dbeta_brent(llk, 0.0)
+        gb, gc, _ = oracle.dbeta_grid(llk, 0.0)
+        want = np.sqrt(np.log(2.0)) / s
+        assert abs(c - 1.0) < 0.01
+        assert abs(gc - 1.0) < 0.01 or gc >= 1.0
+        # dCOV/ddbeta = s^2 dbeta exp(t^2) / COV at COV = 1 (t = dbeta s): the
+        # 0.01 COV tolerance maps to a band of about 0.005 / (s sqrt(ln 2)) in dbeta
+        band = 0.012 / (s * np.sqrt(np.log(2.0)))
+        assert abs(b - want) < band + 0.02 * want, (s, b, want)
+        assert abs(b - gb) < 2 * band + 0.01 * want
+        assert np.isclose(w.sum(), 1.0, rtol=1e-12)
+    b1, c1, _ = oracle.dbeta_brent(np.full(200, -5.0) + rng.normal(0, 1e-3, 200), 0.5)
+    assert b1 == 1.0 and c1 < 1.0
+    # the reference's iteration count bounds the search
+    b2, c2, _ = oracle.dbeta_brent(rng.normal(-1000.0, 40.0, 5000), 0.0, maxiter=1)
+    assert 0.0 < b2 < 1.0
+
+
 def test_cov_metric_against_independent_forms(oracle):
     """oracle_cov_eval (COV.cc:334-368): w = exp(dbeta (llk - median)) / sum,
     COV = sd_{n-1}(w) / mean(w), metric (COV - target)^2, and 1e100 when the
