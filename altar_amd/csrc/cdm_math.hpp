@@ -233,16 +233,43 @@ ALTAR_HD inline bool make_side(const double PA[3], const double PB[3], const dou
     return true;
 }
 
+// One coordinate of the 12 vertices (libcdm.py:185-214): P0 the centre's
+// coordinate, (c0, c1, c2) the row of the rotation matrix, full axes A.
+// Contraction into fused multiply-adds is off, so every caller (the
+// geometry, the model's verify, the fused trajectory kernel) rounds alike.
+ALTAR_HD inline void vertex_row(double P0, double Ax, double Ay, double Az, double c0, double c1, double c2,
+                                double out[12])
+{
+#pragma clang fp contract(off)
+    const double p1 = P0 + Ay * c1 / 2.0 + Az * c2 / 2.0;
+    const double p2 = p1 - Ay * c1;
+    const double p3 = p2 - Az * c2;
+    const double p4 = p1 - Az * c2;
+    const double q1 = P0 - Ax * c0 / 2.0 + Az * c2 / 2.0;
+    const double q2 = q1 + Ax * c0;
+    const double q3 = q2 - Az * c2;
+    const double q4 = q1 - Az * c2;
+    const double r1 = P0 + Ax * c0 / 2.0 + Ay * c1 / 2.0;
+    const double r2 = r1 - Ax * c0;
+    const double r3 = r2 - Ay * c1;
+    const double r4 = r1 - Ay * c1;
+    out[0] = p1; out[1] = p2; out[2] = p3; out[3] = p4;
+    out[4] = q1; out[5] = q2; out[6] = q3; out[7] = q4;
+    out[8] = r1; out[9] = r2; out[10] = r3; out[11] = r4;
+}
+
+constexpr double DEG = 3.14159265358979323846 / 180.0;
+
 // FP64 rotation and vertices of libcdm.py:163-214.  V[v] = (x, y, z).
 // Returns true when any vertex is above the free surface.
 ALTAR_HD inline bool make_vertices(double X0, double Y0, double depth, double ax, double ay, double az,
                           double omx, double omy, double omz, double V[12][3])
 {
-    const double d2r = 3.14159265358979323846 / 180.0;
+#pragma clang fp contract(off)
     ax *= 2.0; ay *= 2.0; az *= 2.0;
-    double cx = cos(omx * d2r), sx = sin(omx * d2r);
-    double cy = cos(omy * d2r), sy = sin(omy * d2r);
-    double cz = cos(omz * d2r), sz = sin(omz * d2r);
+    double cx = cos(omx * DEG), sx = sin(omx * DEG);
+    double cy = cos(omy * DEG), sy = sin(omy * DEG);
+    double cz = cos(omz * DEG), sz = sin(omz * DEG);
     double Rx[3][3] = {{1, 0, 0}, {0, cx, sx}, {0, -sx, cx}};
     double Ry[3][3] = {{cy, 0, -sy}, {0, 1, 0}, {sy, 0, cy}};
     double Rz[3][3] = {{cz, sz, 0}, {-sz, cz, 0}, {0, 0, 1}};
@@ -257,25 +284,30 @@ ALTAR_HD inline bool make_vertices(double X0, double Y0, double depth, double ax
         }
     double P0[3] = {X0, Y0, -depth};
     for (int i = 0; i < 3; ++i) {
-        double c0 = R[i][0], c1 = R[i][1], c2 = R[i][2];
-        double p1 = P0[i] + ay * c1 / 2.0 + az * c2 / 2.0;
-        double p2 = p1 - ay * c1;
-        double p3 = p2 - az * c2;
-        double p4 = p1 - az * c2;
-        double q1 = P0[i] - ax * c0 / 2.0 + az * c2 / 2.0;
-        double q2 = q1 + ax * c0;
-        double q3 = q2 - az * c2;
-        double q4 = q1 - az * c2;
-        double r1 = P0[i] + ax * c0 / 2.0 + ay * c1 / 2.0;
-        double r2 = r1 - ax * c0;
-        double r3 = r2 - ay * c1;
-        double r4 = r1 - ay * c1;
-        V[0][i] = p1; V[1][i] = p2; V[2][i] = p3; V[3][i] = p4;
-        V[4][i] = q1; V[5][i] = q2; V[6][i] = q3; V[7][i] = q4;
-        V[8][i] = r1; V[9][i] = r2; V[10][i] = r3; V[11][i] = r4;
+        double row[12];
+        vertex_row(P0[i], ax, ay, az, R[i][0], R[i][1], R[i][2], row);
+        for (int v = 0; v < 12; ++v) V[v][i] = row[v];
     }
     bool above = false;
     for (int v = 0; v < 12; ++v) above = above || (V[v][2] > 0.0);
+    return above;
+}
+
+// The free-surface check of make_vertices without the rest of the geometry:
+// the heights need only the rotation's last row, which is (sin wY,
+// -cos wY sin wX, cos wY cos wX) whatever wZ (Rz leaves z alone).  Products
+// by the rotation matrices' zeros and ones are exact, so make_vertices'
+// row is these three values bit for bit, and the heights come from the same
+// vertex_row: the two functions agree for every source.
+ALTAR_HD inline bool above_free_surface(double depth, double ax, double ay, double az, double omx, double omy)
+{
+#pragma clang fp contract(off)
+    const double cx = cos(omx * DEG), sx = sin(omx * DEG);
+    const double cy = cos(omy * DEG), sy = sin(omy * DEG);
+    double z[12];
+    vertex_row(-depth, ax * 2.0, ay * 2.0, az * 2.0, sy, cy * -sx, cy * cx, z);
+    bool above = false;
+    for (int v = 0; v < 12; ++v) above = above || (z[v] > 0.0);
     return above;
 }
 

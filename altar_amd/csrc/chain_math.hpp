@@ -70,28 +70,34 @@ __device__ inline double surrogate_llk_sum(const float *z, const double *e, cons
 
 // GELU in its tanh form (Hendrycks and Gimpel 2016):
 //   gelu(x) = x/2 (1 + tanh(u)),  u = sqrt(2/pi) (x + 0.044715 x^3),
-// with tanh(u) = 1 - 2 / (1 + exp(2 u)).  Branch-free (exp and reciprocal
-// on the transcendental unit); exp(2u) overflows to +inf for large u, which
-// gives tanh = 1, and underflows to 0 for very negative u, tanh = -1.
-__device__ inline float gelu_tanh_t(float x)
+// evaluated as the identical function x s with s = 1 / (1 + exp(-2 u)) the
+// logistic of 2u, and exp(-2u) = 2^(x (a + b x^2)) with the constants
+//   a = -2 sqrt(2/pi) log2(e),  b = 0.044715 a.
+// Six operations plus exp2 and a reciprocal on the transcendental unit;
+// branch-free: 2^(...) overflows to +inf for very negative x, which gives
+// s = 0, and underflows to 0 for large x, s = 1.
+constexpr float GELU_A = -2.302208198144325f;     // -2 sqrt(2/pi) / ln 2
+constexpr float GELU_B = -0.10294323958f;         // 0.044715 GELU_A
+
+__device__ inline float gelu_logistic(float x)
 {
 #pragma clang fp contract(off)
-    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+    const float e = __builtin_amdgcn_exp2f(x * fmaf(GELU_B, x * x, GELU_A));
+    return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 __device__ inline float gelu(float x)
 {
 #pragma clang fp contract(off)
-    return 0.5f * x * (1.0f + gelu_tanh_t(x));
+    return x * gelu_logistic(x);
 }
 
-// d gelu / dx = (1 + t) / 2 + x/2 (1 - t^2) sqrt(2/pi) (1 + 3 * 0.044715 x^2)
+// d gelu / dx = s + x s (1 - s) 2 u'(x),  u'(x) = sqrt(2/pi) (1 + 3 * 0.044715 x^2)
 __device__ inline float gelu_grad(float x)
 {
 #pragma clang fp contract(off)
-    const float t = gelu_tanh_t(x);
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 0.134145f * x * x);
+    const float s = gelu_logistic(x);
+    return s + x * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 0.134145f * x * x);
 }
 
 }  // namespace altar

@@ -23,7 +23,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int MT = 64, NT = 64, KT = 32;
 constexpr int MLDS = KT + 8;  // row pitch (bf16) of the LDS tiles
 
-// gelu and gelu_grad (tanh form): chain_math.hpp
+// gelu and gelu_grad (the tanh approximation, evaluated as x * logistic(2u)): chain_math.hpp
 
 __device__ inline float bf2f(__bf16 v) { return (float)v; }
 __device__ inline __bf16 f2bf(float v) { return (__bf16)v; }

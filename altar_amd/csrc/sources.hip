@@ -444,10 +444,9 @@ __global__ void cdm_verify_kernel(const double *__restrict__ theta, int C, int P
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const double *t = theta + (size_t)c * P;
-    double V[12][3];
-    bool above = cdm::make_vertices(t[L.xIdx], t[L.xIdx + 1], t[L.dIdx], t[L.aXIdx], t[L.aXIdx + 1],
-                                    t[L.aXIdx + 2], t[L.omegaXIdx], t[L.omegaXIdx + 1], t[L.omegaXIdx + 2], V);
-    if (above) mask[c] += 1;
+    if (cdm::above_free_surface(t[L.dIdx], t[L.aXIdx], t[L.aXIdx + 1], t[L.aXIdx + 2], t[L.omegaXIdx],
+                                t[L.omegaXIdx + 1]))
+        mask[c] += 1;
 }
 
 hipError_t launch_cdm_verify(const double *theta, int C, int P, CdmLayout L, int *mask, hipStream_t st)

@@ -47,7 +47,7 @@ constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
 // tiles), for DOUT = 64 waves 0-3 one 32 x 32 tile each.  The k order is
 // mlp_gemm_kernel's: chunks of 32, within a chunk the lanes' k-sets
 // half * 16 + kk * 8 for kk = 0, 1.
-template <int DIN, int DOUT, bool LAST>
+template <int DIN, int DOUT, bool LAST, int CHUNK = DIN / 16>
 __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt, const __bf16 *__restrict__ W,
                            int wpitch, const float *__restrict__ bias, int lane, int wid)
 {
@@ -61,40 +61,97 @@ __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt,
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[rt][v] = 0.0f;
-    // all of the wave's weight fragments of this layer in flight at once
-    // (DIN / 16 16-byte loads, at most 64 registers), then the MFMA chain
+    // the wave's weight fragments of this layer in flight CHUNK at a time
+    // (16-byte loads; the whole layer at once by default, at most 64
+    // registers), each chunk followed by its MFMAs in k order
     constexpr int KS = DIN / 16;
+    static_assert(KS % CHUNK == 0, "chunks must tile the layer");
     if (active) {
         const __bf16 *wrow = W + (size_t)(col0 + l32) * wpitch + half * 16;
-        bf16x8 b[KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) b[ks] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
+        for (int k0 = 0; k0 < KS; k0 += CHUNK) {
+            bf16x8 b[CHUNK];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
-            bf16x8 a[RT];
+            for (int j = 0; j < CHUNK; ++j) {
+                const int ks = k0 + j;
+                b[j] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
+            }
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) a[rt] = *(const bf16x8 *)(in + (row0 + rt * 32 + l32) * ipitch + k);
+            for (int j = 0; j < CHUNK; ++j) {
+                const int ks = k0 + j;
+                const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
+                bf16x8 a[RT];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[ks], acc[rt], 0, 0, 0);
+                for (int rt = 0; rt < RT; ++rt) a[rt] = *(const bf16x8 *)(in + (row0 + rt * 32 + l32) * ipitch + k);
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+                    acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[j], acc[rt], 0, 0, 0);
+            }
         }
     }
     __syncthreads();  // every wave has read its input
     if (active) {
         const int col = col0 + l32;
         const float bc = bias[col];
+        // one base address per lane, constant offsets per element
+        float *zb = zt + (row0 + 4 * half) * (KOUT + 1) + col;
+        __bf16 *ab = act + (row0 + 4 * half) * APITCH + col;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int row = row0 + rt * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+                const int drow = rt * 32 + 8 * (v >> 2) + (v & 3);
                 const float y = acc[rt][v] + bc;
                 if constexpr (LAST)
-                    zt[row * (KOUT + 1) + col] = y;
+                    zb[drow * (KOUT + 1)] = y;
                 else
-                    act[row * APITCH + col] = (__bf16)gelu(y);
+                    ab[drow * APITCH] = (__bf16)gelu(y);
             }
     }
+    __syncthreads();
+}
+
+// The hidden 256 x 256 layers with the wave's weight fragments held in
+// registers for the whole launch (loaded once by load_fragments): the same
+// MFMA sequence, k order and epilogue as tile_layer<HID, HID, false>, so the
+// same results bit for bit, without reading the weights again each sub-step.
+constexpr int HKS = HID / 16;   // 16-byte fragments per lane of one 256-input layer
+
+__device__ void load_fragments(const __bf16 *__restrict__ W, bf16x8 (&b)[HKS], int lane, int wid)
+{
+    const int half = lane >> 5, l32 = lane & 31;
+    const __bf16 *wrow = W + (size_t)(32 * wid + l32) * HID + half * 16;
+#pragma unroll
+    for (int ks = 0; ks < HKS; ++ks) b[ks] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
+}
+
+__device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const float *__restrict__ bias, int lane,
+                                      int wid)
+{
+    const int half = lane >> 5, l32 = lane & 31;
+    floatx16 acc[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[rt][v] = 0.0f;
+    const __bf16 *arow = act + l32 * APITCH;
+#pragma unroll
+    for (int ks = 0; ks < HKS; ++ks) {
+        const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
+        bf16x8 a[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) a[rt] = *(const bf16x8 *)(arow + rt * 32 * APITCH + k);
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[ks], acc[rt], 0, 0, 0);
+    }
+    __syncthreads();  // every wave has read its input (the activation tile it overwrites)
+    const int col = 32 * wid + l32;
+    const float bc = bias[col];
+    __bf16 *ab = act + 4 * half * APITCH + col;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) ab[(rt * 32 + 8 * (v >> 2) + (v & 3)) * APITCH] = (__bf16)gelu(acc[rt][v] + bc);
     __syncthreads();
 }
 
@@ -132,6 +189,17 @@ __device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
         tile_layer<HID, HID, false>(sh.act, APITCH, sh.act, sh.zt, a.W[l], HID, a.bias[l], lane, wid);
     tile_layer<HID, KOUT, true>(sh.act, APITCH, sh.act, sh.zt, a.W[a.nlayers - 1], HID, a.bias[a.nlayers - 1], lane,
                                 wid);
+}
+
+// network() for a network of four layers, the two hidden 256 x 256 layers
+// from the register fragments w1, w2
+__device__ void network_resident(TrajShared &sh, const TrajArgs &a, const bf16x8 (&w2)[HKS], int lane, int wid)
+{
+    __syncthreads();
+    tile_layer<TRAJ_PMAX, HID, false>(sh.xin, XPITCH, sh.act, sh.zt, a.W[0], KIN, a.bias[0], lane, wid);
+    tile_layer<HID, HID, false, 4>(sh.act, APITCH, sh.act, sh.zt, a.W[1], HID, a.bias[1], lane, wid);
+    hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
+    tile_layer<HID, KOUT, true, 4>(sh.act, APITCH, sh.act, sh.zt, a.W[3], HID, a.bias[3], lane, wid);
 }
 
 // surrogate_llk_kernel's sum for every tile row into sh.sllk: threads
@@ -194,6 +262,11 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         a.post_s[c] = post;
     }
     const int pairs = (P + 1) / 2;
+    // four-layer networks (the surrogate's 10 -> 256 -> 256 -> 256 -> 64) keep
+    // their hidden layers' weights in registers for the whole trajectory
+    const bool resident = a.nlayers == 4;
+    bf16x8 w2[HKS];
+    if (resident) load_fragments(a.W[2], w2, lane, wid);
     for (int sub = 0; sub < a.substeps; ++sub) {
         const uint32_t step2 = a.step2 + (uint32_t)sub;
         // normals (normals_kernel), and the log of each chain's sub-step dice
@@ -234,10 +307,8 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
                 if (sh.kind[j] == 0 && !(sh.pa[j] <= t[j] && t[j] <= sh.pb[j])) bad = 1;
             if (a.has_cdm) {
                 const CdmLayout &cl = a.cdm;
-                double V[12][3];
-                if (cdm::make_vertices(t[cl.xIdx], t[cl.xIdx + 1], t[cl.dIdx], t[cl.aXIdx], t[cl.aXIdx + 1],
-                                       t[cl.aXIdx + 2], t[cl.omegaXIdx], t[cl.omegaXIdx + 1], t[cl.omegaXIdx + 2],
-                                       V))
+                if (cdm::above_free_surface(t[cl.dIdx], t[cl.aXIdx], t[cl.aXIdx + 1], t[cl.aXIdx + 2],
+                                            t[cl.omegaXIdx], t[cl.omegaXIdx + 1]))
                     bad = 1;
             }
             sh.valid[r] = !bad;
@@ -255,7 +326,10 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         }
         __syncthreads();
         write_input(sh, sh.cand, c0, a);
-        network(sh, a, lane, wid);
+        if (resident)
+            network_resident(sh, a, w2, lane, wid);
+        else
+            network(sh, a, lane, wid);
         tile_llk(sh, a);
         // sub-step acceptance (da_traj_step_kernel)
         if (mine) {
