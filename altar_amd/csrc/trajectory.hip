@@ -4,19 +4,26 @@
 // A workgroup owns 64 chains and runs their whole trajectory: the surrogate
 // posterior of the starting state, then `substeps` Metropolis sub-steps of
 // pi* = prior + beta llk_s, each one
-//   normals -> proposal phi + L xi -> prior support and model check ->
-//   prior llk -> MLP (BF16 MFMA, FP32 accumulation) -> surrogate llk ->
-//   accept or keep phi,
+//   normals -> proposal phi + L xi, with each element's support test, prior
+//   term and network input -> MLP (BF16 MFMA, FP32 accumulation) -> the
+//   chain's model check and prior sum (beside the last layer) -> surrogate
+//   llk and accept or keep phi,
 // with the chain state, the activations and the outputs in LDS.  Only the
-// weights come from memory (L2-resident; 300 KB for 10 -> 256 x 3 -> 64).
+// weights come from memory (L2-resident; 300 KB for 10 -> 256 x 3 -> 64);
+// for the four-layer network layers 0 and 2 stay in registers for the whole
+// launch and the others stream in chunks issued one chunk ahead.
 //
 // Every step repeats the arithmetic of the separate kernels it replaces,
 // in the same order: normals_kernel, propose_kernel, prior_verify_kernel,
-// cdm_verify_kernel, restore_kernel, prior_llk_kernel, mlp_input_kernel,
-// mlp_gemm_kernel (the same MFMA sequence over k), surrogate_llk_kernel,
-// posterior_kernel and da_traj_step_kernel.  The two paths therefore give
-// the same trajectories bit for bit, and the surrogate posterior is one
-// function throughout, which the exact stage's ratio relies on.
+// cdm_verify_kernel, prior_llk_kernel, mlp_input_kernel, mlp_gemm_kernel
+// (the same MFMA sequence over k), surrogate_llk_kernel, posterior_kernel
+// and da_traj_step_kernel.  A candidate outside the prior support or above
+// the free surface is rejected, as there; the separate path evaluates the
+// network on the restored state (restore_kernel) for such a chain, which is
+// never read, and here it sees the proposal instead, equally unread.  The
+// two paths therefore give the same trajectories bit for bit, and the
+// surrogate posterior is one function throughout, which the exact stage's
+// ratio relies on.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "chain_math.hpp"
@@ -111,48 +118,153 @@ __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt,
     __syncthreads();
 }
 
-// The hidden 256 x 256 layers with the wave's weight fragments held in
-// registers for the whole launch (loaded once by load_fragments): the same
-// MFMA sequence, k order and epilogue as tile_layer<HID, HID, false>, so the
-// same results bit for bit, without reading the weights again each sub-step.
+// The four-layer network's own path: the same MFMA sequences, k orders and
+// epilogues as tile_layer, so the same results bit for bit.  Layers 0 and 2
+// use weight fragments held in registers for the whole launch
+// (load_fragments); layers 1 and 3 stream theirs in chunks of four, each
+// chunk's loads issued before the MFMAs of the chunk before it, the first
+// chunk by the caller one layer ahead.
 constexpr int HKS = HID / 16;   // 16-byte fragments per lane of one 256-input layer
+constexpr int WCH = 4;          // fragments per streamed chunk
+
+// fragments ks0 .. ks0 + N - 1 of a lane's weight row (the k-sets of
+// tile_layer: 32-wide chunks, within a chunk half * 16 + kk * 8)
+template <int N>
+__device__ inline void load_chunk(const __bf16 *__restrict__ wrow, int ks0, bf16x8 (&b)[N])
+{
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int ks = ks0 + j;
+        b[j] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
+    }
+}
+
+// the lane's weight row of a layer: output column col, k offset half * 16
+__device__ inline const __bf16 *weight_row(const __bf16 *W, int wpitch, int col, int lane)
+{
+    return W + (size_t)col * wpitch + (lane >> 5) * 16;
+}
 
 __device__ void load_fragments(const __bf16 *__restrict__ W, bf16x8 (&b)[HKS], int lane, int wid)
 {
-    const int half = lane >> 5, l32 = lane & 31;
-    const __bf16 *wrow = W + (size_t)(32 * wid + l32) * HID + half * 16;
-#pragma unroll
-    for (int ks = 0; ks < HKS; ++ks) b[ks] = *(const bf16x8 *)(wrow + (ks >> 1) * 32 + (ks & 1) * 8);
+    load_chunk(weight_row(W, HID, 32 * wid + (lane & 31), lane), 0, b);
 }
 
-__device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const float *__restrict__ bias, int lane,
-                                      int wid)
+__device__ inline void zero_acc(floatx16 (&acc)[2])
 {
-    const int half = lane >> 5, l32 = lane & 31;
-    floatx16 acc[2];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[rt][v] = 0.0f;
-    const __bf16 *arow = act + l32 * APITCH;
+}
+
+// MFMAs of fragments ks0 .. ks0 + N - 1 over both 32-row tiles of the input
+template <int N>
+__device__ inline void mfma_chunk(const __bf16 *arow, int apitch, const bf16x8 (&b)[N], int ks0, int half,
+                                  floatx16 (&acc)[2])
+{
 #pragma unroll
-    for (int ks = 0; ks < HKS; ++ks) {
+    for (int j = 0; j < N; ++j) {
+        const int ks = ks0 + j;
         const int k = (ks >> 1) * 32 + half * 16 + (ks & 1) * 8;
         bf16x8 a[2];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) a[rt] = *(const bf16x8 *)(arow + rt * 32 * APITCH + k);
+        for (int rt = 0; rt < 2; ++rt) a[rt] = *(const bf16x8 *)(arow + rt * 32 * apitch + k);
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[ks], acc[rt], 0, 0, 0);
+        for (int rt = 0; rt < 2; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rt], b[j], acc[rt], 0, 0, 0);
     }
-    __syncthreads();  // every wave has read its input (the activation tile it overwrites)
-    const int col = 32 * wid + l32;
+}
+
+// hidden-layer epilogue: bias, GELU, BF16 into the activation tile
+__device__ inline void gelu_store(__bf16 *act, const floatx16 (&acc)[2], const float *__restrict__ bias, int lane,
+                                  int wid)
+{
+    const int col = 32 * wid + (lane & 31);
     const float bc = bias[col];
-    __bf16 *ab = act + 4 * half * APITCH + col;
+    __bf16 *ab = act + 4 * (lane >> 5) * APITCH + col;
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int v = 0; v < 16; ++v) ab[(rt * 32 + 8 * (v >> 2) + (v & 3)) * APITCH] = (__bf16)gelu(acc[rt][v] + bc);
+}
+
+// layer 0 (inputs k < 32) from the register fragments w0; its input tile is
+// not its output tile, so no wait between the MFMAs and the epilogue
+__device__ void input_layer_resident(const __bf16 *xin, __bf16 *act, const bf16x8 (&w0)[2],
+                                     const float *__restrict__ bias, int lane, int wid)
+{
+    floatx16 acc[2];
+    zero_acc(acc);
+    mfma_chunk(xin + (lane & 31) * XPITCH, XPITCH, w0, 0, lane >> 5, acc);
+    gelu_store(act, acc, bias, lane, wid);
     __syncthreads();
+}
+
+// a 256 -> 256 hidden layer from the register fragments b
+__device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const float *__restrict__ bias, int lane,
+                                      int wid)
+{
+    floatx16 acc[2];
+    zero_acc(acc);
+    mfma_chunk(act + (lane & 31) * APITCH, APITCH, b, 0, lane >> 5, acc);
+    __syncthreads();  // every wave has read its input (the activation tile it overwrites)
+    gelu_store(act, acc, bias, lane, wid);
+    __syncthreads();
+}
+
+// a 256 -> 256 hidden layer streaming its weights; pre holds the first
+// chunk (and is reused as a chunk buffer)
+__device__ void hidden_layer_streamed(__bf16 *act, const __bf16 *__restrict__ W, const float *__restrict__ bias,
+                                      bf16x8 (&pre)[WCH], int lane, int wid)
+{
+    const __bf16 *wrow = weight_row(W, HID, 32 * wid + (lane & 31), lane);
+    floatx16 acc[2];
+    zero_acc(acc);
+    bf16x8 nxt[WCH];
+#pragma unroll
+    for (int ch = 0; ch < HKS / WCH; ++ch) {
+        bf16x8(&cur)[WCH] = (ch & 1) ? nxt : pre;
+        bf16x8(&fut)[WCH] = (ch & 1) ? pre : nxt;
+        if (ch + 1 < HKS / WCH) load_chunk(wrow, WCH * (ch + 1), fut);
+        __builtin_amdgcn_sched_barrier(0);  // the next chunk's loads stay ahead of these MFMAs
+        mfma_chunk(act + (lane & 31) * APITCH, APITCH, cur, WCH * ch, lane >> 5, acc);
+    }
+    __syncthreads();  // every wave has read its input (the activation tile it overwrites)
+    gelu_store(act, acc, bias, lane, wid);
+    __syncthreads();
+}
+
+// the 256 -> 64 output layer on waves 0-3, one 32 x 32 tile each (rows
+// 32 ((wid >> 1) & 1), columns 32 (wid & 1)), FP32 into zt; weights
+// streamed as in hidden_layer_streamed.  The caller waits afterwards.
+__device__ void output_layer_streamed(const __bf16 *act, float *zt, const __bf16 *__restrict__ W,
+                                      const float *__restrict__ bias, bf16x8 (&pre)[WCH], int lane, int wid)
+{
+    const int half = lane >> 5, l32 = lane & 31;
+    const int row0 = 32 * ((wid >> 1) & 1), col = 32 * (wid & 1) + l32;
+    const __bf16 *wrow = weight_row(W, HID, col, lane);
+    const __bf16 *arow = act + (row0 + l32) * APITCH;
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    bf16x8 nxt[WCH];
+#pragma unroll
+    for (int ch = 0; ch < HKS / WCH; ++ch) {
+        bf16x8(&cur)[WCH] = (ch & 1) ? nxt : pre;
+        bf16x8(&fut)[WCH] = (ch & 1) ? pre : nxt;
+        if (ch + 1 < HKS / WCH) load_chunk(wrow, WCH * (ch + 1), fut);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < WCH; ++j) {
+            const int ks = WCH * ch + j;
+            const bf16x8 a0 = *(const bf16x8 *)(arow + (ks >> 1) * 32 + half * 16 + (ks & 1) * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, cur[j], acc, 0, 0, 0);
+        }
+    }
+    const float bc = bias[col];
+    float *zb = zt + (row0 + 4 * half) * (KOUT + 1) + col;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) zb[(8 * (v >> 2) + (v & 3)) * (KOUT + 1)] = acc[v] + bc;
 }
 
 struct TrajShared {
@@ -160,13 +272,21 @@ struct TrajShared {
     __bf16 xin[TT * XPITCH];   // network input; columns P.. and rows past C stay zero
     float zt[TT * (KOUT + 1)];
     double phi[TT * PP], cand[TT * PP], xi[TT * PP];
+    double pterm[TT * PP];     // prior term of each candidate element
     double L[TRAJ_PMAX * TRAJ_PMAX];
     double lo[TRAJ_PMAX], hi[TRAJ_PMAX], pa[TRAJ_PMAX], pb[TRAJ_PMAX], pconst[TRAJ_PMAX];
     int kind[TRAJ_PMAX];
     double e[KOUT], s[KOUT];
     double prior_phi[TT], post_phi[TT], cprior[TT], sllk[TT], logu[TT];
     int valid[TT];
+    int ebad[TT];              // some element of the candidate is outside the prior support
 };
+
+// network input of one parameter value (mlp_input_kernel's scaling)
+__device__ inline __bf16 input_value(double x, double lo, double hi)
+{
+    return (__bf16)(float)(2.0 * (x - lo) / (hi - lo) - 1.0);
+}
 
 // network input of the tile's rows from a chain tile: the first P columns
 // of the rows below C (the rest of xin is zeroed once per launch)
@@ -174,8 +294,7 @@ __device__ void write_input(TrajShared &sh, const double *rows, int c0, const Tr
 {
     for (int it = threadIdx.x; it < TT * a.P; it += THREADS) {
         const int r = it / a.P, j = it % a.P;
-        if (c0 + r < a.C)
-            sh.xin[r * XPITCH + j] = (__bf16)(float)(2.0 * (rows[r * PP + j] - sh.lo[j]) / (sh.hi[j] - sh.lo[j]) - 1.0);
+        if (c0 + r < a.C) sh.xin[r * XPITCH + j] = input_value(rows[r * PP + j], sh.lo[j], sh.hi[j]);
     }
 }
 
@@ -191,20 +310,9 @@ __device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
                                 wid);
 }
 
-// network() for a network of four layers, the two hidden 256 x 256 layers
-// from the register fragments w1, w2
-__device__ void network_resident(TrajShared &sh, const TrajArgs &a, const bf16x8 (&w2)[HKS], int lane, int wid)
-{
-    __syncthreads();
-    tile_layer<TRAJ_PMAX, HID, false>(sh.xin, XPITCH, sh.act, sh.zt, a.W[0], KIN, a.bias[0], lane, wid);
-    tile_layer<HID, HID, false, 4>(sh.act, APITCH, sh.act, sh.zt, a.W[1], HID, a.bias[1], lane, wid);
-    hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
-    tile_layer<HID, KOUT, true, 4>(sh.act, APITCH, sh.act, sh.zt, a.W[3], HID, a.bias[3], lane, wid);
-}
-
-// surrogate_llk_kernel's sum for every tile row into sh.sllk: threads
-// 4 r .. 4 r + 3 take the four partial sums of row r, the first adds them in order
-__device__ void tile_llk(TrajShared &sh, const TrajArgs &a)
+// surrogate_llk_kernel's four partial sums of tile row tid / 4 on threads
+// tid < 256, gathered on the row's first thread (the others' result is unused)
+__device__ inline double row_llk(const TrajShared &sh, const TrajArgs &a)
 {
     const int tid = threadIdx.x, lane = tid & 63;
     double part = 0.0;
@@ -212,7 +320,78 @@ __device__ void tile_llk(TrajShared &sh, const TrajArgs &a)
         part = surrogate_llk_part(sh.zt + (tid >> 2) * (KOUT + 1), sh.e, sh.s, a.K, tid & 3);
     const int src = lane < 61 ? lane : 60;  // lanes 61-63 take no part in a combination
     const double p1 = __shfl(part, src + 1, 64), p2 = __shfl(part, src + 2, 64), p3 = __shfl(part, src + 3, 64);
-    if (tid < SLLK_PARTS * TT && (tid & 3) == 0) sh.sllk[tid >> 2] = surrogate_llk_combine(part, p1, p2, p3, a.cperp, a.norm);
+    return surrogate_llk_combine(part, p1, p2, p3, a.cperp, a.norm);
+}
+
+// every tile row's surrogate llk into sh.sllk
+__device__ void tile_llk(TrajShared &sh, const TrajArgs &a)
+{
+    const int tid = threadIdx.x;
+    const double llk = row_llk(sh, a);
+    if (tid < SLLK_PARTS * TT && (tid & 3) == 0) sh.sllk[tid >> 2] = llk;
+    __syncthreads();
+}
+
+// proposal (propose_kernel) of every element of the tile, with its support
+// test (prior_verify_kernel: a failure marks the row in ebad), its prior
+// term (prior_llk_kernel's summand) and its network input (mlp_input_kernel)
+__device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0)
+{
+    const int P = a.P;
+    for (int it = threadIdx.x; it < TT * P; it += THREADS) {
+        const int rr = it / P, i = it % P;
+        if (c0 + rr >= a.C) continue;
+        const double v = propose_elem(sh.phi[rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
+        sh.cand[rr * PP + i] = v;
+        const bool uniform = sh.kind[i] == 0;
+        if (uniform && !(sh.pa[i] <= v && v <= sh.pb[i])) sh.ebad[rr] = 1;
+        sh.pterm[rr * PP + i] = uniform ? sh.pconst[i] : gauss_log_term(v, sh.pa[i], sh.pb[i], sh.pconst[i]);
+        sh.xin[rr * XPITCH + i] = input_value(v, sh.lo[i], sh.hi[i]);
+    }
+}
+
+// the rest of the candidate's checks for tile row r (cdm_verify_kernel) and,
+// for a valid candidate, its prior (prior_llk_kernel's sum, in order)
+__device__ void check_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
+{
+    if (c0 + r >= a.C) return;
+    const double *t = sh.cand + r * PP;
+    int bad = sh.ebad[r];
+    sh.ebad[r] = 0;  // for the next sub-step
+    if (a.has_cdm) {
+        const CdmLayout &cl = a.cdm;
+        if (cdm::above_free_surface(t[cl.dIdx], t[cl.aXIdx], t[cl.aXIdx + 1], t[cl.aXIdx + 2], t[cl.omegaXIdx],
+                                    t[cl.omegaXIdx + 1]))
+            bad = 1;
+    }
+    sh.valid[r] = !bad;
+    if (!bad) sh.cprior[r] = ordered_sum(sh.pterm + r * PP, a.P);
+}
+
+// surrogate llk of every tile row and the sub-step's acceptance
+// (da_traj_step_kernel) on the row's first thread, the row's owner
+__device__ void llk_accept(TrajShared &sh, const TrajArgs &a, int c0, unsigned &n_acc, unsigned &n_rej,
+                           unsigned &n_unl, int &moved)
+{
+    const int tid = threadIdx.x, P = a.P;
+    const double cdata_s = row_llk(sh, a);
+    const int r = tid >> 2;
+    if (tid < SLLK_PARTS * TT && (tid & 3) == 0 && c0 + r < a.C) {
+        if (!sh.valid[r]) {
+            ++n_rej;
+        } else {
+            const double cp = tempered(sh.cprior[r], a.beta, cdata_s);
+            if (sh.logu[r] <= cp - sh.post_phi[r]) {
+                ++n_acc;
+                moved = 1;
+                for (int j = 0; j < P; ++j) sh.phi[r * PP + j] = sh.cand[r * PP + j];
+                sh.prior_phi[r] = sh.cprior[r];
+                sh.post_phi[r] = cp;
+            } else {
+                ++n_unl;
+            }
+        }
+    }
     __syncthreads();
 }
 
@@ -235,6 +414,7 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         // the data-independent term of prior_llk_kernel's density
         sh.pconst[j] = a.kind[j] == 0 ? log(1.0 / (a.pb[j] - a.pa[j])) : log(a.pb[j] * sqrt(2.0 * PI));
     }
+    if (tid < TT) sh.ebad[tid] = 0;
     for (int k = tid; k < a.K; k += THREADS) {
         sh.e[k] = a.e[k];
         sh.s[k] = a.s[k];
@@ -249,24 +429,27 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
     write_input(sh, sh.phi, c0, a);
     network(sh, a, lane, wid);
     tile_llk(sh, a);
-    const int r = tid;  // the chain of a thread of wavefront 0
-    const int c = c0 + r;
-    const bool mine = tid < TT && c < a.C;
-    int moved = 0;
-    unsigned n_acc = 0, n_rej = 0, n_unl = 0;
-    if (mine) {
-        const double llk = sh.sllk[r];
-        const double post = tempered(a.prior_theta[c], a.beta, llk);
+    if (tid < TT && c0 + tid < a.C) {
+        const int r = tid, c = c0 + tid;
+        const double post = tempered(a.prior_theta[c], a.beta, sh.sllk[r]);
         sh.prior_phi[r] = a.prior_theta[c];
         sh.post_phi[r] = post;
         a.post_s[c] = post;
     }
+    // the chain of tile row r is owned (accepted, counted) by thread 4 r
+    int moved = 0;
+    unsigned n_acc = 0, n_rej = 0, n_unl = 0;
     const int pairs = (P + 1) / 2;
     // four-layer networks (the surrogate's 10 -> 256 -> 256 -> 256 -> 64) keep
-    // their hidden layers' weights in registers for the whole trajectory
-    const bool resident = a.nlayers == 4;
-    bf16x8 w2[HKS];
-    if (resident) load_fragments(a.W[2], w2, lane, wid);
+    // layers 0 and 2 in registers for the whole trajectory
+    const bool four = a.nlayers == 4;
+    bf16x8 w0[2], w2[HKS];
+    if (four) {
+        load_chunk(weight_row(a.W[0], KIN, 32 * wid + (lane & 31), lane), 0, w0);
+        load_fragments(a.W[2], w2, lane, wid);
+    }
+    const __bf16 *w1row = weight_row(a.W[1], HID, 32 * wid + (lane & 31), lane);
+    const __bf16 *w3row = weight_row(a.W[a.nlayers - 1], HID, 32 * (wid & 1) + (lane & 31), lane);
     for (int sub = 0; sub < a.substeps; ++sub) {
         const uint32_t step2 = a.step2 + (uint32_t)sub;
         // normals (normals_kernel), and the log of each chain's sub-step dice
@@ -291,87 +474,46 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
             if (2 * w + 1 < P) sh.xi[rr * PP + 2 * w + 1] = rad * sn;
         }
         __syncthreads();
-        // proposal (propose_kernel)
-        for (int it = tid; it < TT * P; it += THREADS) {
-            const int rr = it / P, i = it % P;
-            if (c0 + rr >= a.C) continue;
-            sh.cand[rr * PP + i] = propose_elem(sh.phi[rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
-        }
+        propose_tile(sh, a, c0);
         __syncthreads();
-        // support and model check, restore, prior (prior_verify_kernel,
-        // cdm_verify_kernel, restore_kernel, prior_llk_kernel)
-        if (mine) {
-            double *t = sh.cand + r * PP;
-            int bad = 0;
-            for (int j = 0; j < P; ++j)
-                if (sh.kind[j] == 0 && !(sh.pa[j] <= t[j] && t[j] <= sh.pb[j])) bad = 1;
-            if (a.has_cdm) {
-                const CdmLayout &cl = a.cdm;
-                if (cdm::above_free_surface(t[cl.dIdx], t[cl.aXIdx], t[cl.aXIdx + 1], t[cl.aXIdx + 2],
-                                            t[cl.omegaXIdx], t[cl.omegaXIdx + 1]))
-                    bad = 1;
-            }
-            sh.valid[r] = !bad;
-            if (bad)
-                for (int j = 0; j < P; ++j) t[j] = sh.phi[r * PP + j];
-            double acc = 0.0;
-            for (int j = 0; j < P; ++j) {
-                if (sh.kind[j] == 0) {
-                    acc += sh.pconst[j];
-                } else {
-                    acc += gauss_log_term(t[j], sh.pa[j], sh.pb[j], sh.pconst[j]);
-                }
-            }
-            sh.cprior[r] = acc;
-        }
-        __syncthreads();
-        write_input(sh, sh.cand, c0, a);
-        if (resident)
-            network_resident(sh, a, w2, lane, wid);
-        else
+        if (four) {
+            bf16x8 pre[WCH];
+            load_chunk(w1row, 0, pre);  // layer 1's first chunk, in flight during layer 0
+            input_layer_resident(sh.xin, sh.act, w0, a.bias[0], lane, wid);
+            hidden_layer_streamed(sh.act, a.W[1], a.bias[1], pre, lane, wid);
+            if (wid < 4) load_chunk(w3row, 0, pre);  // layer 3's, during layer 2
+            hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
+            if (wid < 4)
+                output_layer_streamed(sh.act, sh.zt, a.W[3], a.bias[3], pre, lane, wid);
+            else if (wid == 4)
+                check_row(sh, a, lane, c0);  // beside the output layer
+            __syncthreads();
+        } else {
+            if (wid == 4) check_row(sh, a, lane, c0);
             network(sh, a, lane, wid);
-        tile_llk(sh, a);
-        // sub-step acceptance (da_traj_step_kernel)
-        if (mine) {
-            const double cdata_s = sh.sllk[r];
-            const double cp = tempered(sh.cprior[r], a.beta, cdata_s);
-            if (!sh.valid[r]) {
-                ++n_rej;
-            } else {
-                if (sh.logu[r] <= cp - sh.post_phi[r]) {
-                    ++n_acc;
-                    moved = 1;
-                    for (int j = 0; j < P; ++j) sh.phi[r * PP + j] = sh.cand[r * PP + j];
-                    sh.prior_phi[r] = sh.cprior[r];
-                    sh.post_phi[r] = cp;
-                } else {
-                    ++n_unl;
-                }
-            }
         }
-        __syncthreads();
+        llk_accept(sh, a, c0, n_acc, n_rej, n_unl, moved);
     }
     for (int it = tid; it < TT * P; it += THREADS) {
         const int rr = it / P, j = it % P;
         if (c0 + rr < a.C) a.phi[(size_t)(c0 + rr) * P + j] = sh.phi[rr * PP + j];
     }
-    if (mine) {
+    if (tid < SLLK_PARTS * TT && (tid & 3) == 0 && c0 + (tid >> 2) < a.C) {
+        const int r = tid >> 2, c = c0 + r;
         a.prior_phi[c] = sh.prior_phi[r];
         a.post_phi[c] = sh.post_phi[r];
         a.moved[c] = moved;
     }
-    if (wid == 0) {
-        unsigned long long s0 = n_acc, s1 = n_rej, s2 = n_unl;
-        for (int off = 32; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off, 64);
-            s1 += __shfl_xor(s1, off, 64);
-            s2 += __shfl_xor(s2, off, 64);
-        }
-        if (lane == 0) {
-            if (s0) atomicAdd(&a.counters[0], s0);
-            if (s1) atomicAdd(&a.counters[1], s1);
-            if (s2) atomicAdd(&a.counters[2], s2);
-        }
+    unsigned long long s0 = n_acc, s1 = n_rej, s2 = n_unl;
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    if (lane == 0) {
+        if (s0) atomicAdd(&a.counters[0], s0);
+        if (s1) atomicAdd(&a.counters[1], s1);
+        if (s2) atomicAdd(&a.counters[2], s2);
     }
 }
 
