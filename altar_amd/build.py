@@ -11,6 +11,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaltar_amd.so")
 SOURCES = ["sources.hip", "gemm.hip", "sampler.hip", "mlp.hip", "cov.hip", "trajectory.hip", "capi.hip"]
+
+# Per-source options.  trajectory.hip: the fused kernel keeps network weights
+# in registers across its sub-step loop, and machine-level loop-invariant code
+# motion would otherwise pin the FP64 constants of log and sincos in vector
+# registers for the whole loop, leaving too few for the weights.
+EXTRA = {"trajectory.hip": ["-mllvm", "-disable-machine-licm"]}
 ARCH = os.environ.get("ALTAR_OFFLOAD_ARCH", "gfx950")
 
 
@@ -34,7 +40,7 @@ def build(verbose=False, jobs=4):
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [hipcc()] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + flags() + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -43,10 +43,10 @@ __device__ inline double gauss_log_term(double t, double mean, double sd, double
 }
 
 // surrogate data llk from K network outputs z (unit stride):
-// norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2.  The sum is taken as four
-// interleaved partial sums (k = q mod 4, each over k in order), added to
-// cperp in the order q = 0, 1, 2, 3, so that four threads can share a row.
-constexpr int SLLK_PARTS = 4;
+// norm - (cperp + sum_k (e_k + s_k z_k)^2) / 2.  The sum is taken as eight
+// interleaved partial sums (part q over k = q mod 8, in order of k), added
+// to cperp in the order q = 0 .. 7, so that eight threads can share a row.
+constexpr int SLLK_PARTS = 8;
 
 __device__ inline double surrogate_llk_part(const float *z, const double *e, const double *s, int K, int q)
 {
@@ -60,22 +60,49 @@ __device__ inline double surrogate_llk_part(const float *z, const double *e, con
     return acc;
 }
 
-__device__ inline double surrogate_llk_combine(double p0, double p1, double p2, double p3, double cperp, double norm)
+// surrogate_llk_part for K <= KMAX with a fixed trip count, the loads of
+// every term issued ahead of the arithmetic: the same terms in the same
+// order, each k >= K adding +0.0 to a sum of squares that is never -0.0,
+// which leaves it unchanged.  z, e and s must be readable up to KMAX - 1
+// (the values past K are not used).
+template <int KMAX>
+__device__ inline double surrogate_llk_part_upto(const float *z, const double *e, const double *s, int K, int q)
+{
+#pragma clang fp contract(off)
+    constexpr int N = KMAX / SLLK_PARTS;
+    double ev[N], sv[N];
+    float zv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        ev[i] = e[q + SLLK_PARTS * i];
+        sv[i] = s[q + SLLK_PARTS * i];
+        zv[i] = z[q + SLLK_PARTS * i];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double t = ev[i] + sv[i] * (double)zv[i];
+        acc += q + SLLK_PARTS * i < K ? t * t : 0.0;
+    }
+    return acc;
+}
+
+__device__ inline double surrogate_llk_combine(const double (&p)[SLLK_PARTS], double cperp, double norm)
 {
 #pragma clang fp contract(off)
     double acc = cperp;
-    acc += p0;
-    acc += p1;
-    acc += p2;
-    acc += p3;
+#pragma unroll
+    for (int q = 0; q < SLLK_PARTS; ++q) acc += p[q];
     return norm - 0.5 * acc;
 }
 
 __device__ inline double surrogate_llk_sum(const float *z, const double *e, const double *s, int K, double cperp,
                                            double norm)
 {
-    return surrogate_llk_combine(surrogate_llk_part(z, e, s, K, 0), surrogate_llk_part(z, e, s, K, 1),
-                                 surrogate_llk_part(z, e, s, K, 2), surrogate_llk_part(z, e, s, K, 3), cperp, norm);
+    double p[SLLK_PARTS];
+#pragma unroll
+    for (int q = 0; q < SLLK_PARTS; ++q) p[q] = surrogate_llk_part(z, e, s, K, q);
+    return surrogate_llk_combine(p, cperp, norm);
 }
 
 // GELU in its tanh form (Hendrycks and Gimpel 2016):

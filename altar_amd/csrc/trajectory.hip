@@ -10,8 +10,8 @@
 //   llk and accept or keep phi,
 // with the chain state, the activations and the outputs in LDS.  Only the
 // weights come from memory (L2-resident; 300 KB for 10 -> 256 x 3 -> 64);
-// for the four-layer network layers 0 and 2 stay in registers for the whole
-// launch and the others stream in chunks issued one chunk ahead.
+// for the four-layer network the weights of layers 0-2 stay in registers
+// for the whole launch and layer 3's stream in chunks issued ahead of use.
 //
 // Every step repeats the arithmetic of the separate kernels it replaces,
 // in the same order: normals_kernel, propose_kernel, prior_verify_kernel,
@@ -119,11 +119,9 @@ __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt,
 }
 
 // The four-layer network's own path: the same MFMA sequences, k orders and
-// epilogues as tile_layer, so the same results bit for bit.  Layers 0 and 2
-// use weight fragments held in registers for the whole launch
-// (load_fragments); layers 1 and 3 stream theirs in chunks of four, each
-// chunk's loads issued before the MFMAs of the chunk before it, the first
-// chunk by the caller one layer ahead.
+// epilogues as tile_layer, so the same results bit for bit.  Layers 0, 1
+// and 2 use weight fragments held in registers for the whole launch
+// (load_chunk, load_fragments); layer 3 streams its own.
 constexpr int HKS = HID / 16;   // 16-byte fragments per lane of one 256-input layer
 constexpr int WCH = 4;          // fragments per streamed chunk
 
@@ -212,31 +210,11 @@ __device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const
     __syncthreads();
 }
 
-// a 256 -> 256 hidden layer streaming its weights; pre holds the first
-// chunk (and is reused as a chunk buffer)
-__device__ void hidden_layer_streamed(__bf16 *act, const __bf16 *__restrict__ W, const float *__restrict__ bias,
-                                      bf16x8 (&pre)[WCH], int lane, int wid)
-{
-    const __bf16 *wrow = weight_row(W, HID, 32 * wid + (lane & 31), lane);
-    floatx16 acc[2];
-    zero_acc(acc);
-    bf16x8 nxt[WCH];
-#pragma unroll
-    for (int ch = 0; ch < HKS / WCH; ++ch) {
-        bf16x8(&cur)[WCH] = (ch & 1) ? nxt : pre;
-        bf16x8(&fut)[WCH] = (ch & 1) ? pre : nxt;
-        if (ch + 1 < HKS / WCH) load_chunk(wrow, WCH * (ch + 1), fut);
-        __builtin_amdgcn_sched_barrier(0);  // the next chunk's loads stay ahead of these MFMAs
-        mfma_chunk(act + (lane & 31) * APITCH, APITCH, cur, WCH * ch, lane >> 5, acc);
-    }
-    __syncthreads();  // every wave has read its input (the activation tile it overwrites)
-    gelu_store(act, acc, bias, lane, wid);
-    __syncthreads();
-}
-
 // the 256 -> 64 output layer on waves 0-3, one 32 x 32 tile each (rows
 // 32 ((wid >> 1) & 1), columns 32 (wid & 1)), FP32 into zt; weights
-// streamed as in hidden_layer_streamed.  The caller waits afterwards.
+// streamed in chunks of four fragments, each chunk's loads issued before the
+// MFMAs of the chunk before it, the first chunk (pre, reused as a chunk
+// buffer) by the caller one layer ahead.  The caller waits afterwards.
 __device__ void output_layer_streamed(const __bf16 *act, float *zt, const __bf16 *__restrict__ W,
                                       const float *__restrict__ bias, bf16x8 (&pre)[WCH], int lane, int wid)
 {
@@ -267,20 +245,35 @@ __device__ void output_layer_streamed(const __bf16 *act, float *zt, const __bf16
     for (int v = 0; v < 16; ++v) zb[(8 * (v >> 2) + (v & 3)) * (KOUT + 1)] = acc[v] + bc;
 }
 
+// The small per-parameter and per-row arrays come first, below 64 KB, so
+// that their addresses fold into the LDS instructions' offset field.
 struct TrajShared {
-    __bf16 act[TT * APITCH];
-    __bf16 xin[TT * XPITCH];   // network input; columns P.. and rows past C stay zero
-    float zt[TT * (KOUT + 1)];
-    double phi[TT * PP], cand[TT * PP], xi[TT * PP];
-    double pterm[TT * PP];     // prior term of each candidate element
-    double L[TRAJ_PMAX * TRAJ_PMAX];
     double lo[TRAJ_PMAX], hi[TRAJ_PMAX], pa[TRAJ_PMAX], pb[TRAJ_PMAX], pconst[TRAJ_PMAX];
     int kind[TRAJ_PMAX];
     double e[KOUT], s[KOUT];
     double prior_phi[TT], post_phi[TT], cprior[TT], sllk[TT], logu[TT];
     int valid[TT];
+    int cur[TT];
     int ebad[TT];              // some element of the candidate is outside the prior support
+    double L[TRAJ_PMAX * TRAJ_PMAX];
+    // two chain tiles: row r's state phi is tile cur[r], its candidate the
+    // other one, so that an acceptance flips cur[r] instead of copying
+    double chain[2][TT * PP], xi[TT * PP];
+    double pterm[TT * PP];     // prior term of each candidate element
+    float zt[TT * (KOUT + 1)];
+    __bf16 xin[TT * XPITCH];   // network input; columns P.. and rows past C stay zero
+    __bf16 act[TT * APITCH];
 };
+
+// x as a value the compiler cannot treat as loop-invariant: the address
+// arithmetic derived from it inside the sub-step loop is redone each
+// sub-step instead of being held in registers across the loop, which leaves
+// them to the weights
+__device__ inline int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 // network input of one parameter value (mlp_input_kernel's scaling)
 __device__ inline __bf16 input_value(double x, double lo, double hi)
@@ -290,9 +283,9 @@ __device__ inline __bf16 input_value(double x, double lo, double hi)
 
 // network input of the tile's rows from a chain tile: the first P columns
 // of the rows below C (the rest of xin is zeroed once per launch)
-__device__ void write_input(TrajShared &sh, const double *rows, int c0, const TrajArgs &a)
+__device__ void write_input(TrajShared &sh, const double *rows, int c0, const TrajArgs &a, int tid)
 {
-    for (int it = threadIdx.x; it < TT * a.P; it += THREADS) {
+    for (int it = tid; it < TT * a.P; it += THREADS) {
         const int r = it / a.P, j = it % a.P;
         if (c0 + r < a.C) sh.xin[r * XPITCH + j] = input_value(rows[r * PP + j], sh.lo[j], sh.hi[j]);
     }
@@ -310,39 +303,40 @@ __device__ void network(TrajShared &sh, const TrajArgs &a, int lane, int wid)
                                 wid);
 }
 
-// surrogate_llk_kernel's four partial sums of tile row tid / 4 on threads
-// tid < 256, gathered on the row's first thread (the others' result is unused)
-__device__ inline double row_llk(const TrajShared &sh, const TrajArgs &a)
+// surrogate_llk_kernel's eight partial sums of tile row tid / 8 on threads
+// 8 r .. 8 r + 7, gathered on the row's first thread (the others' result is
+// unused)
+__device__ inline double row_llk(const TrajShared &sh, const TrajArgs &a, int tid)
 {
-    const int tid = threadIdx.x, lane = tid & 63;
-    double part = 0.0;
-    if (tid < SLLK_PARTS * TT)
-        part = surrogate_llk_part(sh.zt + (tid >> 2) * (KOUT + 1), sh.e, sh.s, a.K, tid & 3);
-    const int src = lane < 61 ? lane : 60;  // lanes 61-63 take no part in a combination
-    const double p1 = __shfl(part, src + 1, 64), p2 = __shfl(part, src + 2, 64), p3 = __shfl(part, src + 3, 64);
-    return surrogate_llk_combine(part, p1, p2, p3, a.cperp, a.norm);
+    const int lane = tid & 63;
+    double p[SLLK_PARTS];
+    p[0] = surrogate_llk_part_upto<KOUT>(sh.zt + (tid >> 3) * (KOUT + 1), sh.e, sh.s, a.K, tid & 7);
+    const int src = lane < 64 - SLLK_PARTS ? lane : 64 - SLLK_PARTS;  // the last lanes take no part in a combination
+#pragma unroll
+    for (int q = 1; q < SLLK_PARTS; ++q) p[q] = __shfl(p[0], src + q, 64);
+    return surrogate_llk_combine(p, a.cperp, a.norm);
 }
 
 // every tile row's surrogate llk into sh.sllk
-__device__ void tile_llk(TrajShared &sh, const TrajArgs &a)
+__device__ void tile_llk(TrajShared &sh, const TrajArgs &a, int tid)
 {
-    const int tid = threadIdx.x;
-    const double llk = row_llk(sh, a);
-    if (tid < SLLK_PARTS * TT && (tid & 3) == 0) sh.sllk[tid >> 2] = llk;
+    const double llk = row_llk(sh, a, tid);
+    if ((tid & 7) == 0) sh.sllk[tid >> 3] = llk;
     __syncthreads();
 }
 
 // proposal (propose_kernel) of every element of the tile, with its support
 // test (prior_verify_kernel: a failure marks the row in ebad), its prior
 // term (prior_llk_kernel's summand) and its network input (mlp_input_kernel)
-__device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0)
+__device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0, int tid)
 {
     const int P = a.P;
-    for (int it = threadIdx.x; it < TT * P; it += THREADS) {
+    for (int it = tid; it < TT * P; it += THREADS) {
         const int rr = it / P, i = it % P;
         if (c0 + rr >= a.C) continue;
-        const double v = propose_elem(sh.phi[rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
-        sh.cand[rr * PP + i] = v;
+        const int cr = sh.cur[rr];
+        const double v = propose_elem(sh.chain[cr][rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
+        sh.chain[cr ^ 1][rr * PP + i] = v;
         const bool uniform = sh.kind[i] == 0;
         if (uniform && !(sh.pa[i] <= v && v <= sh.pb[i])) sh.ebad[rr] = 1;
         sh.pterm[rr * PP + i] = uniform ? sh.pconst[i] : gauss_log_term(v, sh.pa[i], sh.pb[i], sh.pconst[i]);
@@ -355,7 +349,7 @@ __device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0)
 __device__ void check_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
 {
     if (c0 + r >= a.C) return;
-    const double *t = sh.cand + r * PP;
+    const double *t = sh.chain[sh.cur[r] ^ 1] + r * PP;
     int bad = sh.ebad[r];
     sh.ebad[r] = 0;  // for the next sub-step
     if (a.has_cdm) {
@@ -370,13 +364,12 @@ __device__ void check_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
 
 // surrogate llk of every tile row and the sub-step's acceptance
 // (da_traj_step_kernel) on the row's first thread, the row's owner
-__device__ void llk_accept(TrajShared &sh, const TrajArgs &a, int c0, unsigned &n_acc, unsigned &n_rej,
+__device__ void llk_accept(TrajShared &sh, const TrajArgs &a, int c0, int tid, unsigned &n_acc, unsigned &n_rej,
                            unsigned &n_unl, int &moved)
 {
-    const int tid = threadIdx.x, P = a.P;
-    const double cdata_s = row_llk(sh, a);
-    const int r = tid >> 2;
-    if (tid < SLLK_PARTS * TT && (tid & 3) == 0 && c0 + r < a.C) {
+    const double cdata_s = row_llk(sh, a, tid);
+    const int r = tid >> 3;
+    if ((tid & 7) == 0 && c0 + r < a.C) {
         if (!sh.valid[r]) {
             ++n_rej;
         } else {
@@ -384,7 +377,7 @@ __device__ void llk_accept(TrajShared &sh, const TrajArgs &a, int c0, unsigned &
             if (sh.logu[r] <= cp - sh.post_phi[r]) {
                 ++n_acc;
                 moved = 1;
-                for (int j = 0; j < P; ++j) sh.phi[r * PP + j] = sh.cand[r * PP + j];
+                sh.cur[r] ^= 1;  // the candidate's tile holds the state from now on
                 sh.prior_phi[r] = sh.cprior[r];
                 sh.post_phi[r] = cp;
             } else {
@@ -414,21 +407,24 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         // the data-independent term of prior_llk_kernel's density
         sh.pconst[j] = a.kind[j] == 0 ? log(1.0 / (a.pb[j] - a.pa[j])) : log(a.pb[j] * sqrt(2.0 * PI));
     }
-    if (tid < TT) sh.ebad[tid] = 0;
+    if (tid < TT) {
+        sh.ebad[tid] = 0;
+        sh.cur[tid] = 0;
+    }
     for (int k = tid; k < a.K; k += THREADS) {
         sh.e[k] = a.e[k];
         sh.s[k] = a.s[k];
     }
     for (int it = tid; it < TT * P; it += THREADS) {
         const int r = it / P, j = it % P;
-        if (c0 + r < a.C) sh.phi[r * PP + j] = a.theta[(size_t)(c0 + r) * P + j];
+        if (c0 + r < a.C) sh.chain[0][r * PP + j] = a.theta[(size_t)(c0 + r) * P + j];
     }
     for (int it = tid; it < TT * XPITCH; it += THREADS) sh.xin[it] = (__bf16)0.0f;
     __syncthreads();
     // the surrogate posterior of the starting state
-    write_input(sh, sh.phi, c0, a);
+    write_input(sh, sh.chain[0], c0, a, tid);
     network(sh, a, lane, wid);
-    tile_llk(sh, a);
+    tile_llk(sh, a, tid);
     if (tid < TT && c0 + tid < a.C) {
         const int r = tid, c = c0 + tid;
         const double post = tempered(a.prior_theta[c], a.beta, sh.sllk[r]);
@@ -436,22 +432,23 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         sh.post_phi[r] = post;
         a.post_s[c] = post;
     }
-    // the chain of tile row r is owned (accepted, counted) by thread 4 r
+    // the chain of tile row r is owned (accepted, counted) by thread 8 r
     int moved = 0;
     unsigned n_acc = 0, n_rej = 0, n_unl = 0;
     const int pairs = (P + 1) / 2;
     // four-layer networks (the surrogate's 10 -> 256 -> 256 -> 256 -> 64) keep
-    // layers 0 and 2 in registers for the whole trajectory
+    // the weights of layers 0-2 in registers for the whole trajectory
     const bool four = a.nlayers == 4;
-    bf16x8 w0[2], w2[HKS];
+    bf16x8 w0[2], w1[HKS], w2[HKS];
     if (four) {
+        load_fragments(a.W[1], w1, lane, wid);
         load_chunk(weight_row(a.W[0], KIN, 32 * wid + (lane & 31), lane), 0, w0);
         load_fragments(a.W[2], w2, lane, wid);
     }
-    const __bf16 *w1row = weight_row(a.W[1], HID, 32 * wid + (lane & 31), lane);
-    const __bf16 *w3row = weight_row(a.W[a.nlayers - 1], HID, 32 * (wid & 1) + (lane & 31), lane);
     for (int sub = 0; sub < a.substeps; ++sub) {
         const uint32_t step2 = a.step2 + (uint32_t)sub;
+        // the thread's indices, opaque to loop-invariant code motion
+        const int tid = opaque(threadIdx.x), lane = tid & 63, wid = tid >> 6;
         // normals (normals_kernel), and the log of each chain's sub-step dice
         // (da_traj_step_kernel's), which do not depend on the sub-step's outcome
         for (int it = tid; it < TT * (pairs + 1); it += THREADS) {
@@ -474,14 +471,14 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
             if (2 * w + 1 < P) sh.xi[rr * PP + 2 * w + 1] = rad * sn;
         }
         __syncthreads();
-        propose_tile(sh, a, c0);
+        propose_tile(sh, a, c0, tid);
         __syncthreads();
         if (four) {
             bf16x8 pre[WCH];
-            load_chunk(w1row, 0, pre);  // layer 1's first chunk, in flight during layer 0
             input_layer_resident(sh.xin, sh.act, w0, a.bias[0], lane, wid);
-            hidden_layer_streamed(sh.act, a.W[1], a.bias[1], pre, lane, wid);
-            if (wid < 4) load_chunk(w3row, 0, pre);  // layer 3's, during layer 2
+            hidden_layer_resident(sh.act, w1, a.bias[1], lane, wid);
+            if (wid < 4) load_chunk(weight_row(a.W[3], HID, 32 * (wid & 1) + (lane & 31), lane), 0, pre);  // layer 3's,
+                                                                                                      // during layer 2
             hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
             if (wid < 4)
                 output_layer_streamed(sh.act, sh.zt, a.W[3], a.bias[3], pre, lane, wid);
@@ -492,14 +489,14 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
             if (wid == 4) check_row(sh, a, lane, c0);
             network(sh, a, lane, wid);
         }
-        llk_accept(sh, a, c0, n_acc, n_rej, n_unl, moved);
+        llk_accept(sh, a, c0, tid, n_acc, n_rej, n_unl, moved);
     }
     for (int it = tid; it < TT * P; it += THREADS) {
         const int rr = it / P, j = it % P;
-        if (c0 + rr < a.C) a.phi[(size_t)(c0 + rr) * P + j] = sh.phi[rr * PP + j];
+        if (c0 + rr < a.C) a.phi[(size_t)(c0 + rr) * P + j] = sh.chain[sh.cur[rr]][rr * PP + j];
     }
-    if (tid < SLLK_PARTS * TT && (tid & 3) == 0 && c0 + (tid >> 2) < a.C) {
-        const int r = tid >> 2, c = c0 + r;
+    if ((tid & 7) == 0 && c0 + (tid >> 3) < a.C) {
+        const int r = tid >> 3, c = c0 + r;
         a.prior_phi[c] = sh.prior_phi[r];
         a.post_phi[c] = sh.post_phi[r];
         a.moved[c] = moved;
