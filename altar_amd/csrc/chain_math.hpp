@@ -18,15 +18,6 @@ __device__ inline double propose_elem(double t, const double *l, const double *x
     return t + acc;
 }
 
-// sum of p[0 .. n-1] from 0.0, in index order
-__device__ inline double ordered_sum(const double *p, int n)
-{
-    double acc = 0.0;
-#pragma unroll 4
-    for (int j = 0; j < n; ++j) acc += p[j];
-    return acc;
-}
-
 // tempered posterior: prior + beta data
 __device__ inline double tempered(double prior, double beta, double data)
 {

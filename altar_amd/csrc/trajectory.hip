@@ -4,14 +4,14 @@
 // A workgroup owns 64 chains and runs their whole trajectory: the surrogate
 // posterior of the starting state, then `substeps` Metropolis sub-steps of
 // pi* = prior + beta llk_s, each one
-//   normals -> proposal phi + L xi, with each element's support test, prior
-//   term and network input -> MLP (BF16 MFMA, FP32 accumulation) -> the
-//   chain's model check and prior sum (beside the last layer) -> surrogate
-//   llk and accept or keep phi,
+//   normals -> proposal phi + L xi, with each element's support test and
+//   network input -> MLP (BF16 MFMA, FP32 accumulation) -> the chain's
+//   model check and prior (beside the last layer) -> surrogate llk and
+//   accept or keep phi,
 // with the chain state, the activations and the outputs in LDS.  Only the
 // weights come from memory (L2-resident; 300 KB for 10 -> 256 x 3 -> 64);
 // for the four-layer network the weights of layers 0-2 stay in registers
-// for the whole launch and layer 3's stream in chunks issued ahead of use.
+// for the whole launch and layer 3's in LDS.
 //
 // Every step repeats the arithmetic of the separate kernels it replaces,
 // in the same order: normals_kernel, propose_kernel, prior_verify_kernel,
@@ -45,6 +45,7 @@ constexpr int KOUT = 64;           // padded output width
 constexpr int APITCH = HID + 8;    // bf16 row pitch of the activation tile
 constexpr int XPITCH = KIN + 8;    // bf16 row pitch of the network-input tile
 constexpr int PP = TRAJ_PMAX + 1;  // row pitch (doubles) of the chain tiles
+constexpr int W3PITCH = HID + 8;   // bf16 row pitch of the output layer's weights in LDS
 
 // One dense layer on the 64-row tile: in (64 x DIN bf16, row pitch ipitch)
 // times W^T (W: DOUT x DIN bf16, row-major) plus bias; GELU into act (hidden
@@ -121,9 +122,8 @@ __device__ void tile_layer(const __bf16 *in, int ipitch, __bf16 *act, float *zt,
 // The four-layer network's own path: the same MFMA sequences, k orders and
 // epilogues as tile_layer, so the same results bit for bit.  Layers 0, 1
 // and 2 use weight fragments held in registers for the whole launch
-// (load_chunk, load_fragments); layer 3 streams its own.
+// (load_chunk, load_fragments), layer 3 a copy of its weights in LDS.
 constexpr int HKS = HID / 16;   // 16-byte fragments per lane of one 256-input layer
-constexpr int WCH = 4;          // fragments per streamed chunk
 
 // fragments ks0 .. ks0 + N - 1 of a lane's weight row (the k-sets of
 // tile_layer: 32-wide chunks, within a chunk half * 16 + kk * 8)
@@ -211,33 +211,24 @@ __device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const
 }
 
 // the 256 -> 64 output layer on waves 0-3, one 32 x 32 tile each (rows
-// 32 ((wid >> 1) & 1), columns 32 (wid & 1)), FP32 into zt; weights
-// streamed in chunks of four fragments, each chunk's loads issued before the
-// MFMAs of the chunk before it, the first chunk (pre, reused as a chunk
-// buffer) by the caller one layer ahead.  The caller waits afterwards.
-__device__ void output_layer_streamed(const __bf16 *act, float *zt, const __bf16 *__restrict__ W,
-                                      const float *__restrict__ bias, bf16x8 (&pre)[WCH], int lane, int wid)
+// 32 ((wid >> 1) & 1), columns 32 (wid & 1)), FP32 into zt, its weights
+// from the copy w3 in LDS (row pitch W3PITCH).  The caller waits afterwards.
+__device__ void output_layer_lds(const __bf16 *act, float *zt, const __bf16 *w3, const float *__restrict__ bias,
+                                 int lane, int wid)
 {
     const int half = lane >> 5, l32 = lane & 31;
     const int row0 = 32 * ((wid >> 1) & 1), col = 32 * (wid & 1) + l32;
-    const __bf16 *wrow = weight_row(W, HID, col, lane);
-    const __bf16 *arow = act + (row0 + l32) * APITCH;
+    const __bf16 *wrow = w3 + col * W3PITCH + half * 16;
+    const __bf16 *arow = act + (row0 + l32) * APITCH + half * 16;
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-    bf16x8 nxt[WCH];
 #pragma unroll
-    for (int ch = 0; ch < HKS / WCH; ++ch) {
-        bf16x8(&cur)[WCH] = (ch & 1) ? nxt : pre;
-        bf16x8(&fut)[WCH] = (ch & 1) ? pre : nxt;
-        if (ch + 1 < HKS / WCH) load_chunk(wrow, WCH * (ch + 1), fut);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < WCH; ++j) {
-            const int ks = WCH * ch + j;
-            const bf16x8 a0 = *(const bf16x8 *)(arow + (ks >> 1) * 32 + half * 16 + (ks & 1) * 8);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, cur[j], acc, 0, 0, 0);
-        }
+    for (int ks = 0; ks < HKS; ++ks) {
+        const int k = (ks >> 1) * 32 + (ks & 1) * 8;
+        const bf16x8 a0 = *(const bf16x8 *)(arow + k);
+        const bf16x8 b0 = *(const bf16x8 *)(wrow + k);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
     }
     const float bc = bias[col];
     float *zb = zt + (row0 + 4 * half) * (KOUT + 1) + col;
@@ -259,10 +250,10 @@ struct TrajShared {
     // two chain tiles: row r's state phi is tile cur[r], its candidate the
     // other one, so that an acceptance flips cur[r] instead of copying
     double chain[2][TT * PP], xi[TT * PP];
-    double pterm[TT * PP];     // prior term of each candidate element
     float zt[TT * (KOUT + 1)];
     __bf16 xin[TT * XPITCH];   // network input; columns P.. and rows past C stay zero
     __bf16 act[TT * APITCH];
+    __bf16 w3[KOUT * W3PITCH];  // the four-layer network's output-layer weights
 };
 
 // x as a value the compiler cannot treat as loop-invariant: the address
@@ -326,8 +317,8 @@ __device__ void tile_llk(TrajShared &sh, const TrajArgs &a, int tid)
 }
 
 // proposal (propose_kernel) of every element of the tile, with its support
-// test (prior_verify_kernel: a failure marks the row in ebad), its prior
-// term (prior_llk_kernel's summand) and its network input (mlp_input_kernel)
+// test (prior_verify_kernel: a failure marks the row in ebad) and its
+// network input (mlp_input_kernel)
 __device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0, int tid)
 {
     const int P = a.P;
@@ -337,15 +328,12 @@ __device__ void propose_tile(TrajShared &sh, const TrajArgs &a, int c0, int tid)
         const int cr = sh.cur[rr];
         const double v = propose_elem(sh.chain[cr][rr * PP + i], sh.L + i * P, sh.xi + rr * PP, i);
         sh.chain[cr ^ 1][rr * PP + i] = v;
-        const bool uniform = sh.kind[i] == 0;
-        if (uniform && !(sh.pa[i] <= v && v <= sh.pb[i])) sh.ebad[rr] = 1;
-        sh.pterm[rr * PP + i] = uniform ? sh.pconst[i] : gauss_log_term(v, sh.pa[i], sh.pb[i], sh.pconst[i]);
+        if (sh.kind[i] == 0 && !(sh.pa[i] <= v && v <= sh.pb[i])) sh.ebad[rr] = 1;
         sh.xin[rr * XPITCH + i] = input_value(v, sh.lo[i], sh.hi[i]);
     }
 }
 
-// the rest of the candidate's checks for tile row r (cdm_verify_kernel) and,
-// for a valid candidate, its prior (prior_llk_kernel's sum, in order)
+// the rest of tile row r's checks (cdm_verify_kernel): sh.valid[r]
 __device__ void check_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
 {
     if (c0 + r >= a.C) return;
@@ -359,7 +347,19 @@ __device__ void check_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
             bad = 1;
     }
     sh.valid[r] = !bad;
-    if (!bad) sh.cprior[r] = ordered_sum(sh.pterm + r * PP, a.P);
+}
+
+// the prior of tile row r's candidate (prior_llk_kernel's sum, in order):
+// sh.cprior[r], used only if the candidate is valid
+__device__ void prior_row(TrajShared &sh, const TrajArgs &a, int r, int c0)
+{
+    if (c0 + r >= a.C) return;
+    const double *t = sh.chain[sh.cur[r] ^ 1] + r * PP;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < a.P; ++j)
+        acc += sh.kind[j] == 0 ? sh.pconst[j] : gauss_log_term(t[j], sh.pa[j], sh.pb[j], sh.pconst[j]);
+    sh.cprior[r] = acc;
 }
 
 // surrogate llk of every tile row and the sub-step's acceptance
@@ -420,6 +420,12 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         if (c0 + r < a.C) sh.chain[0][r * PP + j] = a.theta[(size_t)(c0 + r) * P + j];
     }
     for (int it = tid; it < TT * XPITCH; it += THREADS) sh.xin[it] = (__bf16)0.0f;
+    const bool four = a.nlayers == 4;
+    if (four)  // layer 3's weights, 16 bytes a thread at a time
+        for (int it = tid; it < KOUT * HID / 8; it += THREADS) {
+            const int row = it / (HID / 8), c8 = it % (HID / 8);
+            *(bf16x8 *)(sh.w3 + row * W3PITCH + 8 * c8) = *(const bf16x8 *)(a.W[3] + (size_t)row * HID + 8 * c8);
+        }
     __syncthreads();
     // the surrogate posterior of the starting state
     write_input(sh, sh.chain[0], c0, a, tid);
@@ -438,7 +444,6 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
     const int pairs = (P + 1) / 2;
     // four-layer networks (the surrogate's 10 -> 256 -> 256 -> 256 -> 64) keep
     // the weights of layers 0-2 in registers for the whole trajectory
-    const bool four = a.nlayers == 4;
     bf16x8 w0[2], w1[HKS], w2[HKS];
     if (four) {
         load_fragments(a.W[1], w1, lane, wid);
@@ -474,19 +479,19 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         propose_tile(sh, a, c0, tid);
         __syncthreads();
         if (four) {
-            bf16x8 pre[WCH];
             input_layer_resident(sh.xin, sh.act, w0, a.bias[0], lane, wid);
             hidden_layer_resident(sh.act, w1, a.bias[1], lane, wid);
-            if (wid < 4) load_chunk(weight_row(a.W[3], HID, 32 * (wid & 1) + (lane & 31), lane), 0, pre);  // layer 3's,
-                                                                                                      // during layer 2
             hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
             if (wid < 4)
-                output_layer_streamed(sh.act, sh.zt, a.W[3], a.bias[3], pre, lane, wid);
+                output_layer_lds(sh.act, sh.zt, sh.w3, a.bias[3], lane, wid);
             else if (wid == 4)
                 check_row(sh, a, lane, c0);  // beside the output layer
+            else if (wid == 5)
+                prior_row(sh, a, lane, c0);
             __syncthreads();
         } else {
             if (wid == 4) check_row(sh, a, lane, c0);
+            if (wid == 5) prior_row(sh, a, lane, c0);
             network(sh, a, lane, wid);
         }
         llk_accept(sh, a, c0, tid, n_acc, n_rej, n_unl, moved);
