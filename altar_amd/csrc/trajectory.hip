@@ -202,11 +202,33 @@ __device__ void input_layer_resident(const __bf16 *xin, __bf16 *act, const bf16x
 __device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const float *__restrict__ bias, int lane,
                                       int wid)
 {
-    floatx16 acc[2];
-    zero_acc(acc);
-    mfma_chunk(act + (lane & 31) * APITCH, APITCH, b, 0, lane >> 5, acc);
+    const int half = lane >> 5, l32 = lane & 31;
+    const int col = 32 * wid + l32;
+    const float bc = bias[col];
+    const __bf16 *arow = act + l32 * APITCH + half * 16;
+    floatx16 acc0, acc1;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc0[v] = acc1[v] = 0.0f;
+    // row tile 0, then row tile 1 with row tile 0's epilogue arithmetic
+    // between its MFMAs (each accumulator sees the same MFMA sequence over k)
+#pragma unroll
+    for (int ks = 0; ks < HKS; ++ks) {
+        const bf16x8 a0 = *(const bf16x8 *)(arow + (ks >> 1) * 32 + (ks & 1) * 8);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b[ks], acc0, 0, 0, 0);
+    }
+    __bf16 g0[16];
+#pragma unroll
+    for (int ks = 0; ks < HKS; ++ks) {
+        const bf16x8 a1 = *(const bf16x8 *)(arow + 32 * APITCH + (ks >> 1) * 32 + (ks & 1) * 8);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b[ks], acc1, 0, 0, 0);
+        g0[ks] = (__bf16)gelu(acc0[ks] + bc);
+    }
     __syncthreads();  // every wave has read its input (the activation tile it overwrites)
-    gelu_store(act, acc, bias, lane, wid);
+    __bf16 *ob = act + 4 * half * APITCH + col;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) ob[(8 * (v >> 2) + (v & 3)) * APITCH] = g0[v];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) ob[(32 + 8 * (v >> 2) + (v & 3)) * APITCH] = (__bf16)gelu(acc1[v] + bc);
     __syncthreads();
 }
 
@@ -247,6 +269,7 @@ struct TrajShared {
     int cur[TT];
     int ebad[TT];              // some element of the candidate is outside the prior support
     double L[TRAJ_PMAX * TRAJ_PMAX];
+    float bias[3][HID], bias3[KOUT];  // the four-layer network's biases
     // two chain tiles: row r's state phi is tile cur[r], its candidate the
     // other one, so that an acceptance flips cur[r] instead of copying
     double chain[2][TT * PP], xi[TT * PP];
@@ -421,11 +444,14 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
     }
     for (int it = tid; it < TT * XPITCH; it += THREADS) sh.xin[it] = (__bf16)0.0f;
     const bool four = a.nlayers == 4;
-    if (four)  // layer 3's weights, 16 bytes a thread at a time
+    if (four) {  // layer 3's weights, 16 bytes a thread at a time, and the biases
         for (int it = tid; it < KOUT * HID / 8; it += THREADS) {
             const int row = it / (HID / 8), c8 = it % (HID / 8);
             *(bf16x8 *)(sh.w3 + row * W3PITCH + 8 * c8) = *(const bf16x8 *)(a.W[3] + (size_t)row * HID + 8 * c8);
         }
+        for (int it = tid; it < 3 * HID; it += THREADS) sh.bias[it / HID][it % HID] = a.bias[it / HID][it % HID];
+        if (tid < KOUT) sh.bias3[tid] = a.bias[3][tid];
+    }
     __syncthreads();
     // the surrogate posterior of the starting state
     write_input(sh, sh.chain[0], c0, a, tid);
@@ -479,11 +505,11 @@ __global__ __launch_bounds__(THREADS) void traj_fused_kernel(TrajArgs a)
         propose_tile(sh, a, c0, tid);
         __syncthreads();
         if (four) {
-            input_layer_resident(sh.xin, sh.act, w0, a.bias[0], lane, wid);
-            hidden_layer_resident(sh.act, w1, a.bias[1], lane, wid);
-            hidden_layer_resident(sh.act, w2, a.bias[2], lane, wid);
+            input_layer_resident(sh.xin, sh.act, w0, sh.bias[0], lane, wid);
+            hidden_layer_resident(sh.act, w1, sh.bias[1], lane, wid);
+            hidden_layer_resident(sh.act, w2, sh.bias[2], lane, wid);
             if (wid < 4)
-                output_layer_lds(sh.act, sh.zt, sh.w3, a.bias[3], lane, wid);
+                output_layer_lds(sh.act, sh.zt, sh.w3, sh.bias3, lane, wid);
             else if (wid == 4)
                 check_row(sh, a, lane, c0);  // beside the output layer
             else if (wid == 5)
