@@ -231,12 +231,15 @@ def test_trajectory_linear_gaussian_posterior(cuda, fused):
     assert j_traj > 2.0 * j_plain
 
 
-def test_fused_trajectory_matches_separate_calls(cuda):
+@pytest.mark.parametrize("layers", [4, 3, 5])
+def test_fused_trajectory_matches_separate_calls(cuda, layers):
     """One launch per trajectory (altar_da_trajectory_fused) against the
     separate calls, CDM source (free-surface check), two exact steps of 7
     sub-steps with wide sub-step proposals: states, likelihoods, end points
-    and counters agree bit for bit."""
-    print("case: fused vs separate trajectories, CDM 1000 chains x 512 LOS, 2 steps x 7 sub-steps", flush=True)
+    and counters agree bit for bit.  Four layers take the kernel's
+    register-resident path, three and five its general layer loop."""
+    print(f"case: fused vs separate trajectories, CDM 1000 chains x 512 LOS, 2 steps x 7 sub-steps, "
+          f"{layers} layers", flush=True)
     from altar_amd.cases import cdm_case
     from altar_amd.cdm import CDM
     case = cdm_case(N=512)
@@ -251,7 +254,7 @@ def test_fused_trajectory_matches_separate_calls(cuda):
     w.cool(ann)
     step = w.step
     ann.sampler.prepareSamplingPDF(ann, step)
-    sur = Surrogate(model, K=64, capacity=1024, max_batch=C, seed=2)
+    sur = Surrogate(model, K=64, layers=layers, capacity=1024, max_batch=C, seed=2)
     dev = step.theta.device
     rng = np.random.default_rng(3)
     sur.e_d = torch.from_numpy(rng.normal(0.0, 1.0, 64)).to(dev)
