@@ -36,6 +36,8 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 constexpr int TT = 64;             // chains per workgroup
 constexpr int THREADS = 512;       // 8 wavefronts, two per SIMD
@@ -173,6 +175,25 @@ __device__ inline void mfma_chunk(const __bf16 *arow, int apitch, const bf16x8 (
     }
 }
 
+// two accumulator elements in BF16 with one conversion (round to nearest
+// even, as the scalar conversion)
+__device__ inline bf16x2 to_bf16x2(float a, float b)
+{
+    return __builtin_convertvector((floatx2){a, b}, bf16x2);
+}
+
+// the epilogue of one 32-row tile: bias, GELU, BF16 into rows 8 (v / 4) +
+// (v % 4) of ob (elements v and v + 1 are adjacent rows of one column)
+__device__ inline void gelu_rows(__bf16 *ob, const floatx16 &acc, float bc)
+{
+#pragma unroll
+    for (int v = 0; v < 16; v += 2) {
+        const bf16x2 g = to_bf16x2(gelu(acc[v] + bc), gelu(acc[v + 1] + bc));
+        ob[(8 * (v >> 2) + (v & 3)) * APITCH] = g[0];
+        ob[(8 * (v >> 2) + (v & 3) + 1) * APITCH] = g[1];
+    }
+}
+
 // hidden-layer epilogue: bias, GELU, BF16 into the activation tile
 __device__ inline void gelu_store(__bf16 *act, const floatx16 (&acc)[2], const float *__restrict__ bias, int lane,
                                   int wid)
@@ -180,10 +201,8 @@ __device__ inline void gelu_store(__bf16 *act, const floatx16 (&acc)[2], const f
     const int col = 32 * wid + (lane & 31);
     const float bc = bias[col];
     __bf16 *ab = act + 4 * (lane >> 5) * APITCH + col;
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) ab[(rt * 32 + 8 * (v >> 2) + (v & 3)) * APITCH] = (__bf16)gelu(acc[rt][v] + bc);
+    gelu_rows(ab, acc[0], bc);
+    gelu_rows(ab + 32 * APITCH, acc[1], bc);
 }
 
 // layer 0 (inputs k < 32) from the register fragments w0; its input tile is
@@ -216,19 +235,26 @@ __device__ void hidden_layer_resident(__bf16 *act, const bf16x8 (&b)[HKS], const
         const bf16x8 a0 = *(const bf16x8 *)(arow + (ks >> 1) * 32 + (ks & 1) * 8);
         acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b[ks], acc0, 0, 0, 0);
     }
-    __bf16 g0[16];
+    bf16x2 g0[8];
+    float gprev = 0.0f;
 #pragma unroll
     for (int ks = 0; ks < HKS; ++ks) {
         const bf16x8 a1 = *(const bf16x8 *)(arow + 32 * APITCH + (ks >> 1) * 32 + (ks & 1) * 8);
         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b[ks], acc1, 0, 0, 0);
-        g0[ks] = (__bf16)gelu(acc0[ks] + bc);
+        const float g = gelu(acc0[ks] + bc);
+        if (ks & 1)
+            g0[ks >> 1] = to_bf16x2(gprev, g);
+        else
+            gprev = g;
     }
     __syncthreads();  // every wave has read its input (the activation tile it overwrites)
     __bf16 *ob = act + 4 * half * APITCH + col;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) ob[(8 * (v >> 2) + (v & 3)) * APITCH] = g0[v];
-#pragma unroll
-    for (int v = 0; v < 16; ++v) ob[(32 + 8 * (v >> 2) + (v & 3)) * APITCH] = (__bf16)gelu(acc1[v] + bc);
+    for (int v = 0; v < 16; v += 2) {
+        ob[(8 * (v >> 2) + (v & 3)) * APITCH] = g0[v >> 1][0];
+        ob[(8 * (v >> 2) + (v & 3) + 1) * APITCH] = g0[v >> 1][1];
+    }
+    gelu_rows(ob + 32 * APITCH, acc1, bc);
     __syncthreads();
 }
 
