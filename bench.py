@@ -80,7 +80,7 @@ def parse():
     ap.add_argument("--da-stage-steps", type=int, default=13,
                     help="Metropolis steps per annealing stage before the timed DA stage (training fields are "
                          "collected while beta < 0.1: 13 steps of 16,384 chains fill the 200,000-field store)")
-    ap.add_argument("--da-epochs", type=int, default=100, help="training epochs of the surrogate network")
+    ap.add_argument("--da-epochs", type=int, default=1000, help="training epochs of the surrogate network")
     ap.add_argument("--da-substeps", default="128",
                     help="surrogate sub-steps per exact step in the trajectory leg, or a comma-separated "
                          "list (the first is the reported one; 1 disables the leg)")
