@@ -86,12 +86,12 @@ def parse():
                          "list (the first is the reported one; 1 disables the leg)")
     ap.add_argument("--da-substep-scale", type=float, default=1.6,
                     help="sub-step proposal scale relative to the Metropolis proposal")
-    ap.add_argument("--da-reference-steps", type=int, default=4000,
+    ap.add_argument("--da-reference-steps", type=int, default=6000,
                     help="steps of the surrogate-free plain Metropolis reference run (moments, autocorrelation); "
                          "0 skips the mixing and moment study")
     ap.add_argument("--da-reference-burn", type=int, default=1000,
                     help="burn-in steps of the plain reference run")
-    ap.add_argument("--da-mixing-steps", default="1000,800,800",
+    ap.add_argument("--da-mixing-steps", default="3000,800,800",
                     help="steps of the two-stage, the trajectory and the biased-surrogate trajectory runs of the "
                          "mixing study")
     ap.add_argument("--da-substep-target", type=float, default=0.3,
