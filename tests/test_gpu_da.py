@@ -231,19 +231,20 @@ def test_trajectory_linear_gaussian_posterior(cuda, fused):
     assert j_traj > 2.0 * j_plain
 
 
-@pytest.mark.parametrize("layers", [4, 3, 5])
-def test_fused_trajectory_matches_separate_calls(cuda, layers):
+@pytest.mark.parametrize("layers,C", [(4, 1000), (3, 1000), (5, 1000), (4, 37)])
+def test_fused_trajectory_matches_separate_calls(cuda, layers, C):
     """One launch per trajectory (altar_da_trajectory_fused) against the
     separate calls, CDM source (free-surface check), two exact steps of 7
     sub-steps with wide sub-step proposals: states, likelihoods, end points
     and counters agree bit for bit.  Four layers take the kernel's
-    register-resident path, three and five its general layer loop."""
-    print(f"case: fused vs separate trajectories, CDM 1000 chains x 512 LOS, 2 steps x 7 sub-steps, "
+    register-resident path, three and five its general layer loop; 1,000
+    chains end in a partial 64-chain tile, 37 fill less than one."""
+    print(f"case: fused vs separate trajectories, CDM {C} chains x 512 LOS, 2 steps x 7 sub-steps, "
           f"{layers} layers", flush=True)
     from altar_amd.cases import cdm_case
     from altar_amd.cdm import CDM
     case = cdm_case(N=512)
-    C, P = 1000, 10
+    P = 10
     job = Job(chains=C, steps=1, seed=5)
     model = CDM(case["psets"], case["x"], case["y"], case["los"], case["d"], case["oid"], case["cd"], nu=0.25,
                 job=job).initialize()
@@ -273,7 +274,9 @@ def test_fused_trajectory_matches_separate_calls(cuda, layers):
     (s0, st0, c0, w0), (s1, st1, c1, w1) = runs
     print("exact-stage stats", s0, s1, "sub-steps (accepted, rejected, unlikely)", c0, c1)
     assert s0 == s1 and c0 == c1
-    assert c0[0] > 0 and c0[1] > 0 and c0[2] > 0 and s0[0] > 0
+    assert c0[0] > 0 and s0[0] > 0
+    # every outcome occurs at 1,000 chains (at 37 the rare ones may not)
+    assert (c0[1] > 0 and c0[2] > 0) if C >= 1000 else c0[1] + c0[2] > 0
     for name in ("theta", "prior", "data", "posterior"):
         assert torch.equal(getattr(st0, name), getattr(st1, name)), name
     for name in ("phi", "prior_phi", "post_phi", "post_s"):
