@@ -284,6 +284,53 @@ def test_fused_trajectory_matches_separate_calls(cuda, layers, C):
     assert torch.equal(w0["index"][C:], w1["index"][C:])
 
 
+def test_fused_trajectory_matches_separate_calls_linear_p32(cuda):
+    """The fused kernel at its largest parameter count (P = 32, a full
+    network-input tile) with Gaussian priors (the prior terms the CDM case's
+    uniform priors do not reach) on a linear model: states, likelihoods,
+    end points and counters equal the separate calls bit for bit."""
+    print("case: fused vs separate trajectories, linear N=256 P=32 C=700, 2 steps x 7 sub-steps", flush=True)
+    rng = np.random.default_rng(21)
+    N, P, C = 256, 32, 700
+    G = rng.normal(size=(N, P))
+    xy = rng.uniform(0, 2e4, (N, 2))
+    cd = norms.exponential_covariance(xy, 0.5, 5e3) + 0.5 * np.eye(N)
+    d = G @ rng.normal(size=P) + np.linalg.cholesky(cd) @ rng.normal(size=N)
+    job = Job(chains=C, steps=1, seed=9)
+    model = Linear(default_psets(P), G, d, cd, mode="direct", job=job).initialize()
+    ann = Annealer(Metropolis(), COV(), seed=9).initialize(job)
+    ann.model = model
+    w = ann.worker
+    w.start(ann)
+    w.cool(ann)
+    step = w.step
+    ann.sampler.prepareSamplingPDF(ann, step)
+    sur = Surrogate(model, K=64, capacity=1024, max_batch=C, seed=4)
+    dev = step.theta.device
+    sur.e_d = torch.from_numpy(rng.normal(0.0, 1.0, 64)).to(dev)
+    sur.s_d = torch.from_numpy(rng.uniform(0.05, 0.2, 64)).to(dev)
+    sur.cperp = 2.0
+    sur.trained = True
+    runs = []
+    for fused in (False, True):
+        da = DelayedAcceptance(sur, steps=2, substeps=7, substep_scale=1.0, fused=fused)
+        assert da.fused_supported(model, P) == fused
+        da.sigma_chol = ann.sampler.sigma_chol
+        st = step.clone()
+        stats = da._walk_trajectory(ann, st)
+        torch.cuda.synchronize()
+        runs.append((stats, st, list(da.substep_counts), {k: v.clone() for k, v in da._dws.items()}))
+    (s0, st0, c0, w0), (s1, st1, c1, w1) = runs
+    print("exact-stage stats", s0, s1, "sub-steps (accepted, rejected, unlikely)", c0, c1)
+    assert s0 == s1 and c0 == c1
+    assert c0[0] > 0 and c0[2] > 0 and s0[0] > 0
+    for name in ("theta", "prior", "data", "posterior"):
+        assert torch.equal(getattr(st0, name), getattr(st1, name)), name
+    for name in ("phi", "prior_phi", "post_phi", "post_s"):
+        assert torch.equal(w0[name], w1[name]), name
+    assert torch.equal(w0["index"][C:], w1["index"][C:])
+
+
 def test_substep_scale_tuning(cuda):
     """Pilot trajectories tune the sub-step scale toward the target sub-step
     acceptance and leave the chains where they were."""
