@@ -248,6 +248,11 @@ def test_cdm_llk_full_cd(cuda, oracle):
         print(prec, "max rel llk err", rel.max(), "near-truth abs err", absn.max())
         if tol is not None:
             assert rel.max() < tol, prec
+        # a ragged call (77 chains, the whitening tile's rows past 77 still hold
+        # the previous call's residuals): each chain's llk is its own, bit for bit
+        out77 = torch.empty(77, dtype=torch.float64, device=cuda)
+        src.data_llk(t[:77].contiguous(), out77, prec)
+        assert np.array_equal(out77.cpu().numpy(), got[:77]), prec
 
 
 def test_fields_write_only_the_callers_rows(cuda, oracle):
