@@ -72,6 +72,33 @@ def test_cdm_invariants(oracle):
         oracle.cdm(Xf, Yf, 0.0, 0.0, 0.2, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, op, nu)
 
 
+def test_cdm_rotation_about_the_vertical(oracle):
+    """A half space is symmetric under rotations about a vertical axis.  The
+    oracle builds the source as R = Rz(wZ) Ry(wY) Rx(wX) with Rz(t) =
+    [[cos t, sin t, 0], [-sin t, cos t, 0], [0, 0, 1]], so wZ + 90 turns the
+    whole source by Q = Rz(90) about the vertical through its centre: the
+    field observed at the turned points is the turned field, Q (ue, un), with
+    uv unchanged (and twice, three times for 180 and 270 degrees).  A wrong
+    axis order, a transposed rotation or a sign slip in a horizontal
+    component fails it."""
+    p = KAT_PARAMS
+    g = np.linspace(-6.0, 6.0, 13)
+    X, Y = [a.ravel() for a in np.meshgrid(g, g)]
+    ue, un, uv = oracle.cdm(X, Y, p["X0"], p["Y0"], p["depth"], p["ax"], p["ay"], p["az"], p["omegaX"],
+                            p["omegaY"], p["omegaZ"], p["opening"], p["nu"])
+    scale = max(np.abs(ue).max(), np.abs(un).max(), np.abs(uv).max())
+    dx, dy = X - p["X0"], Y - p["Y0"]
+    e, n = ue, un
+    for turn in (1, 2, 3):
+        dx, dy = dy, -dx          # Q: (x, y) -> (y, -x)
+        e, n = n, -e
+        ur, nr, vr = oracle.cdm(p["X0"] + dx, p["Y0"] + dy, p["X0"], p["Y0"], p["depth"], p["ax"], p["ay"],
+                                p["az"], p["omegaX"], p["omegaY"], p["omegaZ"] + 90.0 * turn, p["opening"], p["nu"])
+        err = max(np.abs(ur - e).max(), np.abs(nr - n).max(), np.abs(vr - uv).max()) / scale
+        print("rotation", 90 * turn, "max error / field scale", err)
+        assert err < 1e-10, turn
+
+
 def test_mogi_closed_form(oracle):
     """Mogi (1958): u = (1 - nu) dV / (pi R^3) [x_o - x_s, y_o - y_s, d] . n - shift."""
     rng = np.random.default_rng(0)
