@@ -1384,20 +1384,25 @@ struct altar_mlp {
     std::vector<DevBuf> pre, act16, actT16, dY16, dYT16;
     DevBuf x16, xT16, out32, lo, hi, loss, tgt, wk;
     DevBuf gpart, cpart;                    // split-k and column-sum partials of the backward pass
-    DevBuf xb, yb;                          // a training batch (altar_mlp_train_epoch)
+    DevBuf xb;                              // a training batch's inputs (altar_mlp_train_epoch)
+    DevBuf adam_segs;                       // the Adam step's tensor table (W, b of each layer)
+    int adam_blocks = 0;
     long step = 0;
     int P = 0;
 };
 
 namespace {
 
-// B input rows; the GEMMs run on B rounded up to the 64-row tile (rows past
-// B hold whatever the buffers held and are never read back)
-void mlp_forward(altar_mlp *h, const double *theta, int B, hipStream_t st)
+// B input rows; the GEMMs run on B rounded up to the 64-row tile, Bp (input
+// rows B..Bp-1 are zero).  With train set, the transposes of the layer
+// inputs (Bp columns) are written beside them for the weight gradients.
+void mlp_forward(altar_mlp *h, const double *theta, int B, hipStream_t st, bool train = false)
 {
-    hip_check(launch_mlp_input(theta, B, h->P, h->lo.as<double>(), h->hi.as<double>(), h->x16.p, h->kin, st),
+    const int Bp = (int)round_up((size_t)B, 64);
+    hip_check(launch_mlp_input(theta, B, h->P, h->lo.as<double>(), h->hi.as<double>(), h->x16.p, h->kin, Bp,
+                               train ? h->xT16.p : nullptr, Bp, st),
               "mlp input");
-    B = (int)round_up((size_t)B, 64);
+    B = Bp;
     const void *X = h->x16.p;
     int ldx = h->kin;
     for (int l = 0; l < h->nlayers; ++l) {
@@ -1408,6 +1413,8 @@ void mlp_forward(altar_mlp *h, const double *theta, int B, hipStream_t st)
         ep.out32 = (l == h->nlayers - 1) ? h->out32.as<float>() : nullptr;
         ep.ldo = h->dout[l];
         ep.act = l < h->nlayers - 1;
+        ep.out16T = (train && l < h->nlayers - 1) ? h->actT16[l].as<__bf16>() : nullptr;
+        ep.ldT = Bp;
         hip_check(launch_mlp_gemm(MLP_EPI_BIAS_ACT, X, ldx, h->W16[l].p, h->din[l], B, h->dout[l], h->din[l], ep, st),
                   "mlp gemm");
         X = h->act16[l].p;
@@ -1426,68 +1433,71 @@ int splitk_slices(int B)
     return 1;
 }
 
-void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int valid, hipStream_t st)
+// Backward pass over B rows (a multiple of 64) after mlp_forward with train
+// set.  Each dY is written with its transpose and the slab partials of its
+// column sums (cpart) by its producer: the loss gradient for the last layer,
+// the backward-data GEMM of layer l + 1 for layer l.  loss (optional)
+// accumulates the batch's loss; it must start at zero.
+void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int valid, float *loss, hipStream_t st)
 {
     const int L = h->nlayers;
     hip_check(launch_mse_grad(h->out32.as<float>(), target, w, B, h->dout[L - 1], h->dout[L - 1], valid,
-                              h->dY16[L - 1].p, h->loss.as<float>(), st),
+                              h->dY16[L - 1].p, h->dYT16[L - 1].p, B, h->cpart.as<float>(), loss, st),
               "mse grad");
     for (int l = L - 1; l >= 0; --l) {
         const int O = h->dout[l], K = h->din[l];
         // weight gradient: dW (O x K) = dY^T (O x B) . X (B x K) -> NT with X^T (K x B)
-        hip_check(launch_transpose_bf16(h->dY16[l].p, B, O, O, h->dYT16[l].p, B, st), "transpose");
-        const void *X = l == 0 ? h->x16.p : h->act16[l - 1].p;
-        void *XT = l == 0 ? h->xT16.p : h->actT16[l - 1].p;
-        hip_check(launch_transpose_bf16(X, B, K, K, XT, B, st), "transpose");
+        const void *XT = l == 0 ? h->xT16.p : h->actT16[l - 1].p;
         hip_check(launch_mlp_gemm_splitk(h->dYT16[l].p, B, XT, B, O, K, B, splitk_slices(B), h->gpart.as<float>(),
                                          h->gW[l].as<float>(), st),
                   "mlp dW");
-        hip_check(launch_colsum(h->dY16[l].p, B, O, O, h->cpart.as<float>(), h->gb[l].as<float>(), st), "mlp db");
+        hip_check(launch_colsum_total(h->cpart.as<float>(), B, O, h->gb[l].as<float>(), st), "mlp db");
         if (l > 0) {
             // data gradient through the previous GELU: dX = (dY . W) * gelu'(pre)
             MlpEpi ed{};
             ed.pre = h->pre[l - 1].as<float>();
             ed.out16 = h->dY16[l - 1].as<__bf16>();
             ed.ldo = K;
+            ed.out16T = h->dYT16[l - 1].as<__bf16>();
+            ed.ldT = B;
+            ed.cpart = h->cpart.as<float>();
             hip_check(launch_mlp_gemm(MLP_EPI_DGELU, h->dY16[l].p, O, h->WT16[l].p, O, B, K, O, ed, st), "mlp dX");
         }
     }
 }
 
-// gradients of the weighted squared error over a batch of B rows (no update)
+// Gradients of the weighted squared error over a batch of B rows (no
+// update), with the batch's targets and the output weights already in tgt
+// and wk.  Columns nout..kout-1 of both are zero from their allocation and
+// nothing writes them; rows B.. of tgt are never read.
+void mlp_staged_gradients(altar_mlp *h, const double *theta_dev, int B, float *loss_dev, hipStream_t st)
+{
+    const int Bp = (int)round_up((size_t)B, 64);
+    // rows B..Bp-1 of the input are zero and their targets are ignored
+    mlp_forward(h, theta_dev, B, st, true);
+    if (loss_dev) hip_check(hipMemsetAsync(h->loss.p, 0, 4, st), "memset");
+    mlp_backward(h, h->tgt.as<float>(), h->wk.as<float>(), Bp, B, loss_dev ? h->loss.as<float>() : nullptr, st);
+    if (loss_dev) hip_check(hipMemcpyAsync(loss_dev, h->loss.p, 4, hipMemcpyDeviceToDevice, st), "loss");
+}
+
+// the same from caller-held targets (B x nout, row pitch ldt) and weights
 void mlp_batch_gradients(altar_mlp *h, const double *theta_dev, const float *target_dev, int ldt, const float *w_dev,
                          int B, float *loss_dev, hipStream_t st)
 {
-    const int Bp = (int)round_up((size_t)B, 64);
-    // rows B..Bp-1 of the input are zeroed, their targets are ignored
-    hip_check(hipMemsetAsync(h->x16.p, 0, (size_t)h->maxB * h->kin * 2, st), "memset");
-    mlp_forward(h, theta_dev, B, st);
-    // targets and weights into the padded (maxB x kout) layout
-    hip_check(hipMemsetAsync(h->tgt.p, 0, (size_t)h->maxB * h->kout * 4, st), "memset");
-    hip_check(hipMemsetAsync(h->wk.p, 0, (size_t)h->kout * 4, st), "memset");
     hip_check(hipMemcpy2DAsync(h->tgt.p, h->kout * 4, target_dev, ldt * 4, h->nout * 4, B,
                                hipMemcpyDeviceToDevice, st),
               "targets");
     hip_check(hipMemcpyAsync(h->wk.p, w_dev, h->nout * 4, hipMemcpyDeviceToDevice, st), "weights");
-    hip_check(hipMemsetAsync(h->loss.p, 0, 4, st), "memset");
-    mlp_backward(h, h->tgt.as<float>(), h->wk.as<float>(), Bp, B, st);
-    if (loss_dev) hip_check(hipMemcpyAsync(loss_dev, h->loss.p, 4, hipMemcpyDeviceToDevice, st), "loss");
+    mlp_staged_gradients(h, theta_dev, B, loss_dev, st);
 }
 
 void mlp_adam_step(altar_mlp *h, float lr, float beta1, float beta2, float eps, hipStream_t st)
 {
     h->step += 1;
     const float c1 = 1.0f - std::pow(beta1, (float)h->step), c2 = 1.0f - std::pow(beta2, (float)h->step);
-    for (int l = 0; l < h->nlayers; ++l) {
-        const int O = h->dout[l], K = h->din[l];
-        hip_check(launch_adam(h->W32[l].as<float>(), h->Wm[l].as<float>(), h->Wv[l].as<float>(),
-                              h->gW[l].as<float>(), O, K, lr, beta1, beta2, eps, c1, c2, h->W16[l].p,
-                              h->WT16[l].p, st),
-                  "adam W");
-        hip_check(launch_adam(h->b32[l].as<float>(), h->bm[l].as<float>(), h->bv[l].as<float>(),
-                              h->gb[l].as<float>(), 1, O, lr, beta1, beta2, eps, c1, c2, nullptr, nullptr, st),
-                  "adam b");
-    }
+    hip_check(launch_adam(h->adam_segs.as<AdamSeg>(), 2 * h->nlayers, h->adam_blocks, lr, beta1, beta2, eps, c1, c2,
+                          st),
+              "adam");
 }
 
 }  // namespace
@@ -1539,6 +1549,19 @@ int altar_mlp_create(altar_mlp **out, int nin, int nhid, int nlayers, int nout, 
         }
         h->gpart.reserve((size_t)SPLITK_MAX * maxOK * 4, true);
         h->cpart.reserve((size_t)colsum_slabs((int)B) * maxO * 4, true);
+        std::vector<AdamSeg> segs;
+        int blocks = 0;
+        for (int l = 0; l < L; ++l) {
+            const int O = h->dout[l], K = h->din[l];
+            segs.push_back({h->W32[l].as<float>(), h->Wm[l].as<float>(), h->Wv[l].as<float>(), h->gW[l].as<float>(),
+                            h->W16[l].as<__bf16>(), h->WT16[l].as<__bf16>(), O, K, blocks});
+            blocks += (O * K + 255) / 256;
+            segs.push_back({h->b32[l].as<float>(), h->bm[l].as<float>(), h->bv[l].as<float>(), h->gb[l].as<float>(),
+                            nullptr, nullptr, 1, O, blocks});
+            blocks += (O + 255) / 256;
+        }
+        upload(h->adam_segs, segs);
+        h->adam_blocks = blocks;
         *out = h;
     });
 }
@@ -1551,7 +1574,7 @@ int altar_mlp_destroy(altar_mlp *h)
                         &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16})
             for (auto &b : *v) b.release();
         for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk, &h->gpart,
-                          &h->cpart, &h->xb, &h->yb})
+                          &h->cpart, &h->xb, &h->adam_segs})
             b->release();
         delete h;
     });
@@ -1666,14 +1689,13 @@ int altar_mlp_train_epoch(altar_mlp *h, const double *theta_store_dev, const flo
                 "bad argument");
         hipStream_t st = S(stream);
         h->xb.reserve((size_t)h->maxB * h->P * 8);
-        h->yb.reserve((size_t)h->maxB * h->nout * 4);
+        hip_check(hipMemcpyAsync(h->wk.p, w_dev, h->nout * 4, hipMemcpyDeviceToDevice, st), "weights");
         for (int b = 0; b < nbatches; ++b) {
             const int *sel = order_dev + (size_t)b * batch;
             hip_check(launch_da_pack(theta_store_dev, sel, batch, h->P, h->xb.as<double>(), st), "batch rows");
-            hip_check(launch_gather_rows_f32(targets_dev, ldt, sel, batch, h->nout, h->yb.as<float>(), h->nout, st),
+            hip_check(launch_gather_rows_f32(targets_dev, ldt, sel, batch, h->nout, h->tgt.as<float>(), h->kout, st),
                       "batch targets");
-            mlp_batch_gradients(h, h->xb.as<double>(), h->yb.as<float>(), h->nout, w_dev, batch,
-                                b == nbatches - 1 ? loss_dev : nullptr, st);
+            mlp_staged_gradients(h, h->xb.as<double>(), batch, b == nbatches - 1 ? loss_dev : nullptr, st);
             const double frac = (double)(it0 + b) / (double)std::max(1, total - 1);
             const double cur = lr_final + 0.5 * ((double)lr - lr_final) * (1.0 + std::cos(M_PI * frac));
             mlp_adam_step(h, (float)cur, beta1, beta2, eps, st);

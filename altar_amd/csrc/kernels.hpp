@@ -84,6 +84,19 @@ struct MlpEpi {
     int ldo;
     int act;              // BIAS_ACT: apply GELU
     size_t split_stride;  // F32 with split-k: slice z is written at out32 + z * split_stride
+    __bf16 *out16T;       // BIAS_ACT, DGELU (optional): the transpose of out16, row pitch ldT
+    int ldT;
+    float *cpart;         // DGELU (optional): column-sum partials of out16 per 64-row slab, pitch ldo
+};
+
+// one tensor of an Adam step (adam_kernel): w, m, v, g are rows x cols FP32;
+// w16 (rows x cols) and wt16 (cols x rows) are optional BF16 copies of w; the
+// segment owns the launch's blocks from first on
+struct AdamSeg {
+    float *w, *m, *v;
+    const float *g;
+    __bf16 *w16, *wt16;
+    int rows, cols, first;
 };
 
 struct AcceptArgs {
@@ -145,21 +158,26 @@ hipError_t launch_gather(const double *theta_old, const double *prior_old, const
 // mlp.hip
 hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
                            const MlpEpi &ep, hipStream_t st);
-hipError_t launch_transpose_bf16(const void *in, int rows, int cols, int ldi, void *out, int ldo, hipStream_t st);
+// dY of the weighted squared error (rows x cols tiles of 64), its transpose
+// dYT (cols x ldT), the 64-row slab partials of its column sums (part,
+// colsum_slabs(rows) x cols) and, if loss is set, the loss added to *loss
 hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, int rows, int cols, int ld,
-                           int valid_rows, void *dy, float *loss, hipStream_t st);
-// db = column sums of dy (rows x cols); part: colsum_slabs(rows) x cols scratch
+                           int valid_rows, void *dy, void *dyT, int ldT, float *part, float *loss, hipStream_t st);
+// db (cols) = the slab partials of part added in slab order
 int colsum_slabs(int rows);
-hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *part, float *db, hipStream_t st);
+hipError_t launch_colsum_total(const float *part, int rows, int cols, float *db, hipStream_t st);
 // out (M x N FP32) = A B^T over K split into S ranges (K % (32 S) == 0); part: S x M x N scratch
 hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb, int M, int N, int K, int S,
                                   float *part, float *out, hipStream_t st);
-hipError_t launch_adam(float *w, float *m, float *v, const float *g, int rows, int cols, float lr, float b1,
-                       float b2, float eps, float c1, float c2, void *w16, void *wt16, hipStream_t st);
+// one Adam step over nseg tensors (table in device memory) in one launch of blocks blocks
+hipError_t launch_adam(const AdamSeg *segs_dev, int nseg, int blocks, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t st);
 hipError_t launch_surrogate_llk(const float *z, int ldz, int C, int K, const double *e, const double *s,
                                 double cperp, double norm, double *llk, hipStream_t st);
+// network input of C rows into x (rows x ldx, rows >= C, zero padded) and,
+// if xT is set, its transpose (ldx x ldT)
 hipError_t launch_mlp_input(const double *theta, int C, int P, const double *lo, const double *hi, void *x, int ldx,
-                            hipStream_t st);
+                            int rows, void *xT, int ldT, hipStream_t st);
 
 // delayed acceptance (sampler.hip)
 hipError_t launch_da_screen(const double *cprior, const double *cdata_s, const double *post_s, const int *mask,

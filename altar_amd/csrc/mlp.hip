@@ -5,10 +5,13 @@
 //   forward            A = X,       B = W      -> bias + GELU epilogue
 //   backward, data     A = dY,      B = W^T    -> times GELU'(pre) epilogue
 //   backward, weights  A = dY^T,    B = X^T    -> FP32 gradient epilogue
-// The transposed copies (W^T, dY^T, X^T) are produced by small transpose
-// kernels and by the optimizer.  Tiles are 64 x 64 x 32 with 4 wavefronts
-// (2 x 2) of one 32 x 32 MFMA tile each; every dimension is padded to the
-// tile sizes by the host.
+// The transposed copies are written beside the originals by their producers:
+// W^T by the optimizer, X^T by the input kernel and the forward epilogue, dY^T
+// by the loss-gradient kernel and the backward-data epilogue (through a
+// transposed tile in LDS, so both copies are written in whole rows).  Tiles
+// are 64 x 64 with 4 wavefronts (2 x 2) of one 32 x 32 MFMA tile each, k in
+// chunks of up to 256 staged whole in LDS (one barrier pair per chunk); every
+// dimension is padded to the tile sizes by the host.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "chain_math.hpp"
@@ -19,47 +22,108 @@ namespace altar {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MT = 64, NT = 64, KT = 32;
-constexpr int MLDS = KT + 8;  // row pitch (bf16) of the LDS tiles
+constexpr int KC = 256;       // k columns staged in LDS at once
+constexpr int CP = KC + 8;    // row pitch (bf16) of the LDS staging tiles
+constexpr int TP = 64 + 8;    // row pitch (bf16) of a transposed 64 x 64 output tile in LDS
 
 // gelu and gelu_grad (the tanh approximation, evaluated as x * logistic(2u)): chain_math.hpp
 
 __device__ inline float bf2f(__bf16 v) { return (float)v; }
 __device__ inline __bf16 f2bf(float v) { return (__bf16)v; }
 
+// From a 64 x 64 bf16 tile held transposed in LDS (T[c * TP + r] = tile row
+// r, column c; NTH threads, 256 or more): writes the transpose to outT (row
+// c0 + c at outT + (c0 + c) * ldT + r0), whole 128-byte rows, if outT is set;
+// and if cpart is set, the tile's column sums to cpart[c0 + c], each the sum
+// of the four 16-row groups ((g0 + g1) + g2) + g3, a group summed row by row
+// from zero.  wp: 256 floats of LDS.  Ends with the block synchronized.
+template <int NTH>
+__device__ inline void transposed_tile_out(const __bf16 *T, float *wp, __bf16 *__restrict__ outT, int ldT, int c0,
+                                           int r0, float *__restrict__ cpart)
+{
+    static_assert(NTH % 256 == 0, "256 threads or a multiple");
+    const int tid = threadIdx.x;
+    if (outT) {
+#pragma unroll
+        for (int j = tid; j < 512; j += NTH) {
+            const int c = j >> 3, r8 = (j & 7) * 8;
+            *(bf16x8 *)(outT + (size_t)(c0 + c) * ldT + r0 + r8) = *(const bf16x8 *)(&T[c * TP + r8]);
+        }
+    }
+    if (cpart) {
+        const int c = tid & 63, g = tid >> 6;
+        if (tid < 256) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) sum += bf2f(T[c * TP + 16 * g + i]);
+            wp[g * 64 + c] = sum;
+        }
+        __syncthreads();
+        if (tid < 64) cpart[c0 + c] = ((wp[c] + wp[64 + c]) + wp[128 + c]) + wp[192 + c];
+    }
+    __syncthreads();
+}
+
 template <int EPI>
 __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict__ A, int lda,
                                                        const __bf16 *__restrict__ B, int ldb, int K,
                                                        MlpEpi ep)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 As[MT * MLDS];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NT * MLDS];
+    // the staging tiles (k chunks of up to KC); after the k loop, the
+    // transposed output tile (64 x TP bf16) and the column-sum partials (256
+    // floats) in the same bytes
+    __shared__ __attribute__((aligned(16))) __bf16 smem[(MT + NT) * CP];
+    static_assert(64 * TP * 2 + 256 * 4 <= (MT + NT) * CP * 2, "epilogue tile exceeds the staging LDS");
+    __bf16 *As = smem, *Bs = smem + MT * CP;
     const int m0 = blockIdx.y * MT, n0 = blockIdx.x * NT;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
-    // staging: 64 rows x 32 bf16 = 256 x 16 bytes: one 16-byte chunk per thread
-    const int lr = tid >> 2, lc = (tid & 3) * 8;
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
     const int half = lane >> 5, l32 = lane & 31;
     // split-k: block z covers k in [z K / gridDim.z, (z + 1) K / gridDim.z)
     const int kspan = K / (int)gridDim.z;
-    const int kbeg = (int)blockIdx.z * kspan;
-    for (int k0 = kbeg; k0 < kbeg + kspan; k0 += KT) {
-        *(bf16x8 *)(&As[lr * MLDS + lc]) = *(const bf16x8 *)(A + (size_t)(m0 + lr) * lda + k0 + lc);
-        *(bf16x8 *)(&Bs[lr * MLDS + lc]) = *(const bf16x8 *)(B + (size_t)(n0 + lr) * ldb + k0 + lc);
-        __syncthreads();
+    const int kbeg = (int)blockIdx.z * kspan, kend = kbeg + kspan;
+    for (int c0 = kbeg; c0 < kend; c0 += KC) {
+        // a chunk of kc columns (a multiple of 32) of 64 rows of A and of B,
+        // 16-byte pieces, consecutive threads along a row; every load of the
+        // chunk is issued before the first LDS store
+        const int kc = min(KC, kend - c0), pr = kc / 8, n = 64 * pr / 256;
+        bf16x8 ra[KC / 32], rb[KC / 32];
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            bf16x8 a = *(const bf16x8 *)(&As[(wm * 32 + l32) * MLDS + half * 16 + kk * 8]);
-            bf16x8 b = *(const bf16x8 *)(&Bs[(wn * 32 + l32) * MLDS + half * 16 + kk * 8]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        for (int i = 0; i < KC / 32; ++i) {
+            if (i < n) {
+                const int j = tid + 256 * i, r = j / pr, c = (j - r * pr) * 8;
+                ra[i] = *(const bf16x8 *)(A + (size_t)(m0 + r) * lda + c0 + c);
+                rb[i] = *(const bf16x8 *)(B + (size_t)(n0 + r) * ldb + c0 + c);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < KC / 32; ++i) {
+            if (i < n) {
+                const int j = tid + 256 * i, r = j / pr, c = (j - r * pr) * 8;
+                *(bf16x8 *)(&As[r * CP + c]) = ra[i];
+                *(bf16x8 *)(&Bs[r * CP + c]) = rb[i];
+            }
+        }
+        __syncthreads();
+        // the same MFMA sequence over k as one 32-column tile at a time
+        for (int k0 = 0; k0 < kc; k0 += KT) {
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                bf16x8 a = *(const bf16x8 *)(&As[(wm * 32 + l32) * CP + k0 + half * 16 + kk * 8]);
+                bf16x8 b = *(const bf16x8 *)(&Bs[(wn * 32 + l32) * CP + k0 + half * 16 + kk * 8]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
         }
         __syncthreads();
     }
     const int col = n0 + wn * 32 + l32;
+    __bf16 o16[16];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int row = m0 + wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
@@ -68,97 +132,109 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict_
             y += ep.bias[col];
             ep.pre[(size_t)row * ep.ldo + col] = y;
             float o = ep.act ? gelu(y) : y;
-            ep.out16[(size_t)row * ep.ldo + col] = f2bf(o);
+            o16[v] = f2bf(o);
+            ep.out16[(size_t)row * ep.ldo + col] = o16[v];
             if (ep.out32) ep.out32[(size_t)row * ep.ldo + col] = o;
         } else if constexpr (EPI == MLP_EPI_F32) {
             ep.out32[(size_t)blockIdx.z * ep.split_stride + (size_t)row * ep.ldo + col] = y;
         } else {  // MLP_EPI_DGELU
             float g = y * gelu_grad(ep.pre[(size_t)row * ep.ldo + col]);
-            ep.out16[(size_t)row * ep.ldo + col] = f2bf(g);
+            o16[v] = f2bf(g);
+            ep.out16[(size_t)row * ep.ldo + col] = o16[v];
         }
     }
-}
-
-// out[c, r] = in[r, c] (bf16), rows x cols input
-__global__ void transpose_bf16_kernel(const __bf16 *__restrict__ in, int rows, int cols, int ldi,
-                                      __bf16 *__restrict__ out, int ldo)
-{
-    __shared__ __bf16 tile[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
-    for (int j = ty; j < 32; j += 8) {
-        const int r = r0 + j, c = c0 + tx;
-        tile[j][tx] = (r < rows && c < cols) ? in[(size_t)r * ldi + c] : f2bf(0.0f);
-    }
-    __syncthreads();
-    for (int j = ty; j < 32; j += 8) {
-        const int c = c0 + j, r = r0 + tx;
-        if (c < cols && r < rows) out[(size_t)c * ldo + r] = tile[tx][j];
+    if constexpr (EPI != MLP_EPI_F32) {
+        if (ep.out16T || ep.cpart) {   // the same for every block of the launch
+            // the k loop ended with a barrier: the staging tiles are free
+            __bf16 *T = smem;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                bf16x4 pk = {o16[4 * q], o16[4 * q + 1], o16[4 * q + 2], o16[4 * q + 3]};
+                *(bf16x4 *)(&T[(wn * 32 + l32) * TP + wm * 32 + 8 * q + 4 * half]) = pk;
+            }
+            __syncthreads();
+            transposed_tile_out<256>(T, (float *)(smem + 64 * TP), ep.out16T, ep.ldT, n0, m0,
+                                ep.cpart ? ep.cpart + (size_t)blockIdx.y * ep.ldo : nullptr);
+        }
     }
 }
 
 // output-layer gradient of the weighted squared error:
-// loss = sum_b sum_k w_k (yhat - y)^2 / B  ->  dY = 2 w_k (yhat - y) / B (bf16);
-// also accumulates the loss (FP32 atomics, for monitoring only)
-__global__ void mse_grad_kernel(const float *__restrict__ yhat, const float *__restrict__ y, const float *__restrict__ w,
-                                int rows, int cols, int ld, int valid_rows, __bf16 *__restrict__ dy,
-                                float *__restrict__ loss)
+// loss = sum_b sum_k w_k (yhat - y)^2 / B  ->  dY = 2 w_k (yhat - y) / B (bf16;
+// rows >= valid_rows give 0), one 64 x 64 tile per block of TILE_THREADS
+// (thread: column tid % 64, rows 4 (tid / 64) .. + 3).  Also writes dY^T
+// (cols x ldT), the tile's column sums into part[blockIdx.y * cols + column]
+// (the slab partials of the bias gradient) and, if loss is set, adds the
+// tile's loss to it (FP32 atomics, for monitoring only).
+constexpr int TILE_THREADS = 1024;
+__global__ __launch_bounds__(TILE_THREADS) void mse_grad_kernel(const float *__restrict__ yhat,
+                                                                const float *__restrict__ y,
+                                                                const float *__restrict__ w, int cols, int ld,
+                                                                int valid_rows, __bf16 *__restrict__ dy,
+                                                                __bf16 *__restrict__ dyT, int ldT,
+                                                                float *__restrict__ part, float *__restrict__ loss)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    float g = 0.0f, l = 0.0f;
-    if (idx < (long)rows * cols) {
-        const int r = (int)(idx / cols), c = (int)(idx % cols);
+    __shared__ __attribute__((aligned(16))) __bf16 T[64 * TP];
+    __shared__ float wp[256];
+    __shared__ float lp[TILE_THREADS / 64];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const float wc = w[c0 + c];
+    float l = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 4 * g + i;
+        float gr = 0.0f;
         if (r < valid_rows) {
-            const float e = yhat[(size_t)r * ld + c] - y[(size_t)r * ld + c];
-            g = 2.0f * w[c] * e / (float)valid_rows;
-            l = w[c] * e * e / (float)valid_rows;
+            const float e = yhat[(size_t)r * ld + c0 + c] - y[(size_t)r * ld + c0 + c];
+            gr = 2.0f * wc * e / (float)valid_rows;
+            l += wc * e * e / (float)valid_rows;
         }
-        dy[(size_t)r * ld + c] = f2bf(g);
+        const __bf16 b = f2bf(gr);
+        dy[(size_t)r * ld + c0 + c] = b;
+        T[c * TP + 4 * g + i] = b;
     }
-    // one atomic per block (the loss is for monitoring only)
-    __shared__ float part[4];
     l = wave_sum(l);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    if ((threadIdx.x & 63) == 0) lp[g] = l;
     __syncthreads();
+    transposed_tile_out<TILE_THREADS>(T, wp, dyT, ldT, c0, r0, part + (size_t)blockIdx.y * cols);
     if (threadIdx.x == 0 && loss) {
-        const float t = part[0] + part[1] + part[2] + part[3];
+        float t = 0.0f;
+        for (int k = 0; k < TILE_THREADS / 64; ++k) t += lp[k];
         if (t != 0.0f) atomicAdd(loss, t);
     }
 }
 
-// db[n] = sum_m dY[m, n] (FP32 out from a bf16 gradient), in two passes.
-// Pass 1: block (x, y) sums rows [64 y, 64 y + 64) of columns
-// [64 x, 64 x + 64): lane = column (coalesced rows), 4 waves of 16 rows each,
-// wave partials added in order into part[y][column].  Pass 2 adds the slabs
-// in order, so the result does not depend on scheduling.
-constexpr int CS_ROWS = 64;              // rows per slab: 4 waves of 16
-__global__ __launch_bounds__(256) void colsum_slab_kernel(const __bf16 *__restrict__ dy, int rows, int cols, int ld,
-                                                          float *__restrict__ part)
+// db[n] = sum_m dY[m, n] (FP32 from the bf16 gradient), in two passes.  Pass
+// 1 is the epilogue of dY's producer (mse_grad_kernel, the backward-data GEMM):
+// per 64-row slab y and column n, part[y][n] adds the slab's four 16-row groups
+// in order, each summed row by row.  Pass 2 (colsum_total_kernel) adds the
+// slabs in order, so the result does not depend on scheduling.
+constexpr int CS_ROWS = 64;              // rows per slab
+
+// t + p[0] + p[stride] + ... + p[(n - 1) stride], added in that order.  The
+// loads of a group of 16 are issued before its adds, so a thread waits for
+// memory once per group rather than once per term.
+__device__ inline float ordered_strided_sum(const float *__restrict__ p, int n, size_t stride)
 {
-    __shared__ float wp[4][64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + lane;
-    const int r0 = blockIdx.y * CS_ROWS + wid * (CS_ROWS / 4);
-    float s = 0.0f;
-    if (c < cols) {
-#pragma unroll 8
-        for (int i = 0; i < CS_ROWS / 4; ++i) {
-            const int r = r0 + i;
-            if (r < rows) s += bf2f(dy[(size_t)r * ld + c]);
-        }
+    float t = 0.0f;
+    int i = 0;
+    for (; i + 16 <= n; i += 16) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = p[(size_t)(i + j) * stride];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) t += v[j];
     }
-    wp[wid][lane] = s;
-    __syncthreads();
-    if (wid == 0 && c < cols) part[(size_t)blockIdx.y * cols + c] = ((wp[0][lane] + wp[1][lane]) + wp[2][lane]) + wp[3][lane];
+    for (; i < n; ++i) t += p[(size_t)i * stride];
+    return t;
 }
 
 __global__ void colsum_total_kernel(const float *__restrict__ part, int slabs, int cols, float *__restrict__ db)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
-    float t = 0.0f;
-    for (int y = 0; y < slabs; ++y) t += part[(size_t)y * cols + c];
-    db[c] = t;
+    db[c] = ordered_strided_sum(part + c, slabs, (size_t)cols);
 }
 
 // out[i] = sum_z part[z stride + i], z in order (split-k partial products)
@@ -166,30 +242,33 @@ __global__ void splitk_sum_kernel(const float *__restrict__ part, int S, size_t 
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float t = 0.0f;
-    for (int z = 0; z < S; ++z) t += part[(size_t)z * stride + i];
-    out[i] = t;
+    out[i] = ordered_strided_sum(part + i, S, stride);
 }
 
-// Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy and
-// its transpose.  W is rows x cols (ld = cols), WT is cols x rows.
-__global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
-                            const float *__restrict__ g, int rows, int cols, float lr, float b1, float b2,
-                            float eps, float c1, float c2, __bf16 *__restrict__ w16, __bf16 *__restrict__ wt16)
+// Adam (Kingma & Ba 2015) on FP32 master weights, every tensor of the
+// network in one launch: segment k (a table in device memory) owns the blocks
+// [first_k, first_{k+1}).  Refreshes the BF16 copy and its transpose where the
+// segment has them.  W is rows x cols (ld = cols), WT is cols x rows.
+__global__ __launch_bounds__(256) void adam_kernel(const AdamSeg *__restrict__ segs, int nseg, float lr, float b1,
+                                                   float b2, float eps, float c1, float c2)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)rows * cols) return;
-    const float gi = g[idx];
-    const float mi = b1 * m[idx] + (1.0f - b1) * gi;
-    const float vi = b2 * v[idx] + (1.0f - b2) * gi * gi;
-    m[idx] = mi;
-    v[idx] = vi;
-    const float wi = w[idx] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-    w[idx] = wi;
-    if (w16) w16[idx] = f2bf(wi);
-    if (wt16) {
-        const int r = (int)(idx / cols), c = (int)(idx % cols);
-        wt16[(size_t)c * rows + r] = f2bf(wi);
+    int k = 0;
+    for (int j = 1; j < nseg; ++j)
+        if ((int)blockIdx.x >= segs[j].first) k = j;
+    const AdamSeg sg = segs[k];
+    const long idx = (long)((int)blockIdx.x - sg.first) * blockDim.x + threadIdx.x;
+    if (idx >= (long)sg.rows * sg.cols) return;
+    const float gi = sg.g[idx];
+    const float mi = b1 * sg.m[idx] + (1.0f - b1) * gi;
+    const float vi = b2 * sg.v[idx] + (1.0f - b2) * gi * gi;
+    sg.m[idx] = mi;
+    sg.v[idx] = vi;
+    const float wi = sg.w[idx] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    sg.w[idx] = wi;
+    if (sg.w16) sg.w16[idx] = f2bf(wi);
+    if (sg.wt16) {
+        const int r = (int)(idx / sg.cols), c = (int)(idx % sg.cols);
+        sg.wt16[(size_t)c * sg.rows + r] = f2bf(wi);
     }
 }
 
@@ -203,17 +282,35 @@ __global__ void surrogate_llk_kernel(const float *__restrict__ z, int ldz, int C
     llk[c] = surrogate_llk_sum(z + (size_t)c * ldz, e, s, K, cperp, norm);
 }
 
-// theta (C x P FP64) -> normalized network input (bf16, C x ldx, zero padded):
-// x' = 2 (x - lo) / (hi - lo) - 1
-__global__ void mlp_input_kernel(const double *__restrict__ theta, int C, int P, const double *__restrict__ lo,
-                                 const double *__restrict__ hi, __bf16 *__restrict__ x, int ldx)
+// theta (C x P FP64) -> normalized network input x (bf16, rows x ldx, both
+// multiples of 64): x' = 2 (x - lo) / (hi - lo) - 1 for row < C and column
+// < P, zero elsewhere; and, if xT is set, its transpose (ldx x ldT).  One
+// 64 x 64 tile per block of TILE_THREADS (4 rows per thread), staged
+// transposed in LDS.
+__global__ __launch_bounds__(TILE_THREADS) void mlp_input_kernel(const double *__restrict__ theta, int C, int P,
+                                                                 const double *__restrict__ lo,
+                                                                 const double *__restrict__ hi,
+                                                                 __bf16 *__restrict__ x, int ldx,
+                                                                 __bf16 *__restrict__ xT, int ldT)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)C * ldx) return;
-    const int c = (int)(idx / ldx), j = (int)(idx % ldx);
-    float v = 0.0f;
-    if (j < P) v = (float)(2.0 * (theta[(size_t)c * P + j] - lo[j]) / (hi[j] - lo[j]) - 1.0);
-    x[idx] = f2bf(v);
+    __shared__ __attribute__((aligned(16))) __bf16 T[64 * TP];
+    __shared__ float wp[256];
+    const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int j = c0 + c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 4 * g + i;
+        float v = 0.0f;
+        if (r < C && j < P) v = (float)(2.0 * (theta[(size_t)r * P + j] - lo[j]) / (hi[j] - lo[j]) - 1.0);
+        const __bf16 b = f2bf(v);
+        x[(size_t)r * ldx + j] = b;
+        T[c * TP + 4 * g + i] = b;
+    }
+    if (xT) {   // the same for every block of the launch
+        __syncthreads();
+        transposed_tile_out<TILE_THREADS>(T, wp, xT, ldT, c0, r0, nullptr);
+    }
 }
 
 hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int ldb, int M, int N, int K,
@@ -230,31 +327,21 @@ hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int l
     return hipGetLastError();
 }
 
-hipError_t launch_transpose_bf16(const void *in, int rows, int cols, int ldi, void *out, int ldo, hipStream_t st)
-{
-    dim3 grid((cols + 31) / 32, (rows + 31) / 32), block(256);
-    hipLaunchKernelGGL(transpose_bf16_kernel, grid, block, 0, st, (const __bf16 *)in, rows, cols, ldi, (__bf16 *)out,
-                       ldo);
-    return hipGetLastError();
-}
-
 hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, int rows, int cols, int ld,
-                           int valid_rows, void *dy, float *loss, hipStream_t st)
+                           int valid_rows, void *dy, void *dyT, int ldT, float *part, float *loss, hipStream_t st)
 {
-    long n = (long)rows * cols;
-    hipLaunchKernelGGL(mse_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, yhat, y, w, rows, cols,
-                       ld, valid_rows, (__bf16 *)dy, loss);
+    if (rows % 64 || cols % 64 || ld < cols || ldT < rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(cols / 64, rows / 64), dim3(TILE_THREADS), 0, st, yhat, y, w, cols, ld,
+                       valid_rows, (__bf16 *)dy, (__bf16 *)dyT, ldT, part, loss);
     return hipGetLastError();
 }
 
 int colsum_slabs(int rows) { return (rows + CS_ROWS - 1) / CS_ROWS; }
 
-hipError_t launch_colsum(const void *dy, int rows, int cols, int ld, float *part, float *db, hipStream_t st)
+hipError_t launch_colsum_total(const float *part, int rows, int cols, float *db, hipStream_t st)
 {
-    const int slabs = colsum_slabs(rows);
-    hipLaunchKernelGGL(colsum_slab_kernel, dim3((cols + 63) / 64, slabs), dim3(256), 0, st, (const __bf16 *)dy, rows,
-                       cols, ld, part);
-    hipLaunchKernelGGL(colsum_total_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, part, slabs, cols, db);
+    hipLaunchKernelGGL(colsum_total_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, part, colsum_slabs(rows), cols,
+                       db);
     return hipGetLastError();
 }
 
@@ -274,12 +361,10 @@ hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb
     return hipGetLastError();
 }
 
-hipError_t launch_adam(float *w, float *m, float *v, const float *g, int rows, int cols, float lr, float b1,
-                       float b2, float eps, float c1, float c2, void *w16, void *wt16, hipStream_t st)
+hipError_t launch_adam(const AdamSeg *segs_dev, int nseg, int blocks, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t st)
 {
-    long n = (long)rows * cols;
-    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, m, v, g, rows, cols, lr,
-                       b1, b2, eps, c1, c2, (__bf16 *)w16, (__bf16 *)wt16);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks), dim3(256), 0, st, segs_dev, nseg, lr, b1, b2, eps, c1, c2);
     return hipGetLastError();
 }
 
@@ -292,11 +377,11 @@ hipError_t launch_surrogate_llk(const float *z, int ldz, int C, int K, const dou
 }
 
 hipError_t launch_mlp_input(const double *theta, int C, int P, const double *lo, const double *hi, void *x, int ldx,
-                            hipStream_t st)
+                            int rows, void *xT, int ldT, hipStream_t st)
 {
-    long n = (long)C * ldx;
-    hipLaunchKernelGGL(mlp_input_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, theta, C, P, lo, hi,
-                       (__bf16 *)x, ldx);
+    if (rows % 64 || ldx % 64 || rows < C || ldx < P || (xT && ldT < rows)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mlp_input_kernel, dim3(ldx / 64, rows / 64), dim3(TILE_THREADS), 0, st, theta, C, P, lo, hi,
+                       (__bf16 *)x, ldx, (__bf16 *)xT, ldT);
     return hipGetLastError();
 }
 

@@ -3,7 +3,8 @@
 * forward: relative error of the network output <= 2e-2 (BF16 operands)
 * gradients: per layer, relative Frobenius error <= 5e-2 and cosine >= 0.995
   against the chain-rule gradients of the same (BF16-rounded) weights
-* training: Adam on a smooth target reduces the loss by 10x
+* training: Adam on a smooth target reduces the loss by 10x; a native epoch
+  equals its batches run one by one, bit for bit
 * delayed acceptance: the screen list, the stage-2 rule and the counters
   match a numpy restatement of Christen and Fox (2005) on the same uniforms
 """
@@ -81,6 +82,44 @@ def test_mlp_learns_smooth_map(cuda):
         losses.append(loss.item())
     print("loss first/last", losses[0], losses[-1])
     assert losses[-1] < losses[0] / 10
+
+
+def test_train_epoch_matches_batch_calls(cuda):
+    """One native epoch (batch gather, gradients, Adam with the cosine rate)
+    gives the weights of the same batches run through altar_mlp_backward and
+    altar_mlp_adam one by one, bit for bit; its loss is the last batch's."""
+    import math
+    rng = np.random.default_rng(4)
+    N, batch, nb, lr, lr_final, total = 3000, 640, 4, 2e-3, 1e-4, 12
+    theta = torch.from_numpy(rng.uniform(-5, 5, (N, 10))).to(cuda)
+    A = rng.normal(size=(10, 64))
+    target = torch.from_numpy(np.tanh(theta.cpu().numpy() / 5.0 @ A).astype(np.float32)).to(cuda)
+    w = torch.from_numpy((rng.random(64) / 32).astype(np.float32)).to(cuda)
+    order = rng.permutation(N)[: nb * batch].astype(np.int32)
+    a, _, lo, hi = setup_net(3)
+    b, _, _, _ = setup_net(3)
+    for net in (a, b):   # a few steps first, so that Adam's moments are not zero
+        for it in range(2):
+            net.backward(theta[it * 100:(it + 1) * 100 + 37], target[it * 100:(it + 1) * 100 + 37], w)
+            net.adam(1e-3)
+    loss_a = torch.zeros(1, dtype=torch.float32, device=cuda)
+    order_d = torch.from_numpy(order).to(cuda)
+    _lib.call("altar_mlp_train_epoch", a.handle, _lib.ptr(theta), _lib.ptr(target), 64, _lib.ptr(w),
+              _lib.ptr(order_d), nb, batch, 4, total, lr, lr_final, 0.9, 0.999, 1e-8, _lib.ptr(loss_a), _lib.stream())
+    loss_b = torch.zeros(1, dtype=torch.float32, device=cuda)
+    for i in range(nb):
+        sel = torch.from_numpy(order[i * batch:(i + 1) * batch].astype(np.int64)).to(cuda)
+        b.backward(theta[sel].contiguous(), target[sel].contiguous(), w, loss_b)
+        # the epoch takes both rates as FP32 and forms its schedule in FP64
+        hi_, lo_ = float(np.float32(lr)), float(np.float32(lr_final))
+        frac = (4 + i) / (total - 1)
+        b.adam(lo_ + 0.5 * (hi_ - lo_) * (1.0 + math.cos(math.pi * frac)))
+    torch.cuda.synchronize()
+    for l in range(4):
+        for x, y in zip(a.layer(l), b.layer(l)):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"layer {l} differs"
+    print("epoch loss", loss_a.item(), "last batch loss", loss_b.item())
+    assert abs(loss_a.item() - loss_b.item()) <= 1e-6 * abs(loss_b.item())
 
 
 def test_delayed_acceptance_rule(cuda, oracle):
