@@ -262,14 +262,16 @@ class Surrogate:
         tr = np.asarray(tr_idx)
         batch = min(batch, self.max_batch)
         nb = max(1, len(tr) // batch)
-        loss = torch.zeros(1, dtype=torch.float32, device=dev)
+        # Each epoch's last-batch loss stays on the device until the end, so the
+        # host draws the next epoch's order while the device runs this one.
+        losses = torch.zeros(epochs, dtype=torch.float32, device=dev)
         total = epochs * nb
         for ep in range(epochs):
             order = torch.from_numpy(rng.permutation(tr).astype(np.int32)).to(dev)
             _lib.call("altar_mlp_train_epoch", self.mlp.handle, _lib.ptr(self.theta), _lib.ptr(target_all), self.K,
                       _lib.ptr(w_d), _lib.ptr(order), nb, batch, ep * nb, total, float(lr), float(lr_final), 0.9,
-                      0.999, 1e-8, _lib.ptr(loss), _lib.stream())
-            self.history.append(float(loss.item()))
+                      0.999, 1e-8, _lib.ptr(losses[ep:ep + 1]), _lib.stream())
+        self.history.extend(float(v) for v in losses.cpu().numpy())
         self.trained = True
         self.val_idx = val_idx
         self.validation_rmse = self.validate_stored(val_idx)

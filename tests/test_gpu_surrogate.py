@@ -30,9 +30,10 @@ def setup_net(seed=0, nin=10, nhid=256, nout=64):
     return net, layers, lo, hi
 
 
-def test_mlp_forward_and_gradients(cuda):
+@pytest.mark.parametrize("nhid", [256, 320])
+def test_mlp_forward_and_gradients(cuda, nhid):
     from oracle import mlp as om
-    net, layers, lo, hi = setup_net()
+    net, layers, lo, hi = setup_net(nhid=nhid)
     rng = np.random.default_rng(0)
     B = 200   # not a multiple of the 64-row tile
     theta = rng.uniform(-5, 5, (B, 10))
