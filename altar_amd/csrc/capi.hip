@@ -1383,10 +1383,12 @@ struct altar_mlp {
     std::vector<DevBuf> W32, Wm, Wv, W16, WT16, b32, bm, bv, gW, gb;
     std::vector<DevBuf> pre, act16, actT16, dY16, dYT16;
     DevBuf x16, xT16, out32, lo, hi, loss, tgt, wk;
-    DevBuf gpart, cpart;                    // split-k and column-sum partials of the backward pass
     DevBuf xb;                              // a training batch's inputs (altar_mlp_train_epoch)
     DevBuf adam_segs;                       // the Adam step's tensor table (W, b of each layer)
     int adam_blocks = 0;
+    std::vector<DevBuf> cparts, gparts;     // per-layer column-sum and split-k partials of the backward pass
+    DevBuf wgrad_tab, reduce_tab;           // segment tables of the grouped weight-gradient and sum launches
+    int tab_rows = -1, wgrad_blocks = 0, reduce_blocks = 0;   // the batch rows the tables were built for
     long step = 0;
     int P = 0;
 };
@@ -1433,37 +1435,63 @@ int splitk_slices(int B)
     return 1;
 }
 
+// Segment tables of the grouped weight-gradient GEMM and ordered sums for a
+// batch of B rows (B a multiple of 64); rebuilt when B changes, after the
+// stream has drained, since kernels of an earlier batch may still read them.
+void mlp_backward_tables(altar_mlp *h, int B, hipStream_t st)
+{
+    if (h->tab_rows == B) return;
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    const int L = h->nlayers, S = splitk_slices(B);
+    std::vector<MlpWGradSeg> wg;
+    std::vector<MlpReduceSeg> rd;
+    int wb = 0, rb = 0;
+    for (int l = L - 1; l >= 0; --l) {
+        const int O = h->dout[l], K = h->din[l];
+        const __bf16 *XT = l == 0 ? h->xT16.as<__bf16>() : h->actT16[l - 1].as<__bf16>();
+        wg.push_back({h->dYT16[l].as<__bf16>(), XT, h->gparts[l].as<float>(), O, K, wb});
+        wb += (O / 64) * (K / 64) * S;
+        rd.push_back({h->gparts[l].as<float>(), h->gW[l].as<float>(), (long)O * K, (size_t)O * K, S, rb});
+        rb += (O * K + 255) / 256;
+        rd.push_back({h->cparts[l].as<float>(), h->gb[l].as<float>(), (long)O, (size_t)O, colsum_slabs(B), rb});
+        rb += (O + 255) / 256;
+    }
+    upload(h->wgrad_tab, wg);
+    upload(h->reduce_tab, rd);
+    h->wgrad_blocks = wb;
+    h->reduce_blocks = rb;
+    h->tab_rows = B;
+}
+
 // Backward pass over B rows (a multiple of 64) after mlp_forward with train
 // set.  Each dY is written with its transpose and the slab partials of its
-// column sums (cpart) by its producer: the loss gradient for the last layer,
-// the backward-data GEMM of layer l + 1 for layer l.  loss (optional)
-// accumulates the batch's loss; it must start at zero.
+// column sums by its producer: the loss gradient for the last layer, the
+// backward-data GEMM of layer l + 1 for layer l.  Then one launch forms
+// every layer's split-k weight-gradient partials and one adds them, and the
+// column-sum partials, in order.  loss (optional) accumulates the batch's
+// loss; it must start at zero.
 void mlp_backward(altar_mlp *h, const float *target, const float *w, int B, int valid, float *loss, hipStream_t st)
 {
     const int L = h->nlayers;
+    mlp_backward_tables(h, B, st);
     hip_check(launch_mse_grad(h->out32.as<float>(), target, w, B, h->dout[L - 1], h->dout[L - 1], valid,
-                              h->dY16[L - 1].p, h->dYT16[L - 1].p, B, h->cpart.as<float>(), loss, st),
+                              h->dY16[L - 1].p, h->dYT16[L - 1].p, B, h->cparts[L - 1].as<float>(), loss, st),
               "mse grad");
-    for (int l = L - 1; l >= 0; --l) {
+    for (int l = L - 1; l >= 1; --l) {
         const int O = h->dout[l], K = h->din[l];
-        // weight gradient: dW (O x K) = dY^T (O x B) . X (B x K) -> NT with X^T (K x B)
-        const void *XT = l == 0 ? h->xT16.p : h->actT16[l - 1].p;
-        hip_check(launch_mlp_gemm_splitk(h->dYT16[l].p, B, XT, B, O, K, B, splitk_slices(B), h->gpart.as<float>(),
-                                         h->gW[l].as<float>(), st),
-                  "mlp dW");
-        hip_check(launch_colsum_total(h->cpart.as<float>(), B, O, h->gb[l].as<float>(), st), "mlp db");
-        if (l > 0) {
-            // data gradient through the previous GELU: dX = (dY . W) * gelu'(pre)
-            MlpEpi ed{};
-            ed.pre = h->pre[l - 1].as<float>();
-            ed.out16 = h->dY16[l - 1].as<__bf16>();
-            ed.ldo = K;
-            ed.out16T = h->dYT16[l - 1].as<__bf16>();
-            ed.ldT = B;
-            ed.cpart = h->cpart.as<float>();
-            hip_check(launch_mlp_gemm(MLP_EPI_DGELU, h->dY16[l].p, O, h->WT16[l].p, O, B, K, O, ed, st), "mlp dX");
-        }
+        // data gradient through the previous GELU: dX = (dY . W) * gelu'(pre)
+        MlpEpi ed{};
+        ed.pre = h->pre[l - 1].as<float>();
+        ed.out16 = h->dY16[l - 1].as<__bf16>();
+        ed.ldo = K;
+        ed.out16T = h->dYT16[l - 1].as<__bf16>();
+        ed.ldT = B;
+        ed.cpart = h->cparts[l - 1].as<float>();
+        hip_check(launch_mlp_gemm(MLP_EPI_DGELU, h->dY16[l].p, O, h->WT16[l].p, O, B, K, O, ed, st), "mlp dX");
     }
+    hip_check(launch_mlp_wgrad(h->wgrad_tab.as<MlpWGradSeg>(), L, h->wgrad_blocks, B, splitk_slices(B), B, st),
+              "mlp dW");
+    hip_check(launch_mlp_reduce(h->reduce_tab.as<MlpReduceSeg>(), 2 * L, h->reduce_blocks, st), "mlp sums");
 }
 
 // Gradients of the weighted squared error over a batch of B rows (no
@@ -1542,13 +1570,12 @@ int altar_mlp_create(altar_mlp **out, int nin, int nhid, int nlayers, int nout, 
         h->loss.reserve(4, true);
         h->tgt.reserve(B * h->kout * 4, true);
         h->wk.reserve(h->kout * 4, true);
-        size_t maxOK = 0, maxO = 0;
+        h->cparts.resize(L);
+        h->gparts.resize(L);
         for (int l = 0; l < L; ++l) {
-            maxOK = std::max(maxOK, (size_t)h->dout[l] * h->din[l]);
-            maxO = std::max(maxO, (size_t)h->dout[l]);
+            h->cparts[l].reserve((size_t)colsum_slabs((int)B) * h->dout[l] * 4, true);
+            h->gparts[l].reserve((size_t)SPLITK_MAX * h->dout[l] * h->din[l] * 4, true);
         }
-        h->gpart.reserve((size_t)SPLITK_MAX * maxOK * 4, true);
-        h->cpart.reserve((size_t)colsum_slabs((int)B) * maxO * 4, true);
         std::vector<AdamSeg> segs;
         int blocks = 0;
         for (int l = 0; l < L; ++l) {
@@ -1571,10 +1598,10 @@ int altar_mlp_destroy(altar_mlp *h)
     return guarded([&] {
         if (!h) return;
         for (auto *v : {&h->W32, &h->Wm, &h->Wv, &h->W16, &h->WT16, &h->b32, &h->bm, &h->bv, &h->gW, &h->gb,
-                        &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16})
+                        &h->pre, &h->act16, &h->actT16, &h->dY16, &h->dYT16, &h->cparts, &h->gparts})
             for (auto &b : *v) b.release();
-        for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk, &h->gpart,
-                          &h->cpart, &h->xb, &h->adam_segs})
+        for (DevBuf *b : {&h->x16, &h->xT16, &h->out32, &h->lo, &h->hi, &h->loss, &h->tgt, &h->wk, &h->xb,
+                          &h->adam_segs, &h->wgrad_tab, &h->reduce_tab})
             b->release();
         delete h;
     });

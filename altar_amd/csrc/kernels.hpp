@@ -89,6 +89,23 @@ struct MlpEpi {
     float *cpart;         // DGELU (optional): column-sum partials of out16 per 64-row slab, pitch ldo
 };
 
+// one layer's weight gradient in mlp_wgrad_kernel: the split-k NT product
+// A (M x K) B^T (N x K) into part (S slices of M x N); blocks from first on
+struct MlpWGradSeg {
+    const __bf16 *A, *B;
+    float *part;
+    int M, N, first;
+};
+// one ordered sum of mlp_reduce_kernel: dst[i] = sum over c < count of
+// src[c stride + i], i < n; blocks from first on
+struct MlpReduceSeg {
+    const float *src;
+    float *dst;
+    long n;
+    size_t stride;
+    int count, first;
+};
+
 // one tensor of an Adam step (adam_kernel): w, m, v, g are rows x cols FP32;
 // w16 (rows x cols) and wt16 (cols x rows) are optional BF16 copies of w; the
 // segment owns the launch's blocks from first on
@@ -163,12 +180,13 @@ hipError_t launch_mlp_gemm(int epi, const void *A, int lda, const void *B, int l
 // colsum_slabs(rows) x cols) and, if loss is set, the loss added to *loss
 hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, int rows, int cols, int ld,
                            int valid_rows, void *dy, void *dyT, int ldT, float *part, float *loss, hipStream_t st);
-// db (cols) = the slab partials of part added in slab order
+// 64-row slabs of a batch of rows (the column-sum partials per layer)
 int colsum_slabs(int rows);
-hipError_t launch_colsum_total(const float *part, int rows, int cols, float *db, hipStream_t st);
-// out (M x N FP32) = A B^T over K split into S ranges (K % (32 S) == 0); part: S x M x N scratch
-hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb, int M, int N, int K, int S,
-                                  float *part, float *out, hipStream_t st);
+// every layer's weight gradient (segment table in device memory; K the
+// batch rows, the row pitch of both operands, split into S slices)
+hipError_t launch_mlp_wgrad(const MlpWGradSeg *segs_dev, int nseg, int blocks, int K, int S, int ld, hipStream_t st);
+// several ordered sums in one launch (segment table in device memory)
+hipError_t launch_mlp_reduce(const MlpReduceSeg *segs_dev, int nseg, int blocks, hipStream_t st);
 // one Adam step over nseg tensors (table in device memory) in one launch of blocks blocks
 hipError_t launch_adam(const AdamSeg *segs_dev, int nseg, int blocks, float lr, float b1, float b2, float eps,
                        float c1, float c2, hipStream_t st);

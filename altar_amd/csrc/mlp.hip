@@ -67,27 +67,28 @@ __device__ inline void transposed_tile_out(const __bf16 *T, float *wp, __bf16 *_
     __syncthreads();
 }
 
+// The staging tiles (k chunks of up to KC); after the k loop, the
+// transposed output tile (64 x TP bf16) and the column-sum partials (256
+// floats) in the same bytes.
+constexpr int GEMM_SMEM = (MT + NT) * CP;
+static_assert(64 * TP * 2 + 256 * 4 <= GEMM_SMEM * 2, "epilogue tile exceeds the staging LDS");
+
+// One 64 x 64 output tile (rows m0.., columns n0..) over k in [kbeg, kbeg +
+// kspan): slice z of a split-k product (its partial at out32 + z
+// split_stride), tile row ty (its column-sum partials at row ty of cpart).
 template <int EPI>
-__global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict__ A, int lda,
-                                                       const __bf16 *__restrict__ B, int ldb, int K,
-                                                       MlpEpi ep)
+__device__ inline void mlp_gemm_tile(const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B, int ldb,
+                                     int kbeg, int kspan, int m0, int n0, int z, int ty, const MlpEpi &ep,
+                                     __bf16 *smem)
 {
-    // the staging tiles (k chunks of up to KC); after the k loop, the
-    // transposed output tile (64 x TP bf16) and the column-sum partials (256
-    // floats) in the same bytes
-    __shared__ __attribute__((aligned(16))) __bf16 smem[(MT + NT) * CP];
-    static_assert(64 * TP * 2 + 256 * 4 <= (MT + NT) * CP * 2, "epilogue tile exceeds the staging LDS");
     __bf16 *As = smem, *Bs = smem + MT * CP;
-    const int m0 = blockIdx.y * MT, n0 = blockIdx.x * NT;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     floatx16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
     const int half = lane >> 5, l32 = lane & 31;
-    // split-k: block z covers k in [z K / gridDim.z, (z + 1) K / gridDim.z)
-    const int kspan = K / (int)gridDim.z;
-    const int kbeg = (int)blockIdx.z * kspan, kend = kbeg + kspan;
+    const int kend = kbeg + kspan;
     for (int c0 = kbeg; c0 < kend; c0 += KC) {
         // a chunk of kc columns (a multiple of 32) of 64 rows of A and of B,
         // 16-byte pieces, consecutive threads along a row; every load of the
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict_
             ep.out16[(size_t)row * ep.ldo + col] = o16[v];
             if (ep.out32) ep.out32[(size_t)row * ep.ldo + col] = o;
         } else if constexpr (EPI == MLP_EPI_F32) {
-            ep.out32[(size_t)blockIdx.z * ep.split_stride + (size_t)row * ep.ldo + col] = y;
+            ep.out32[(size_t)z * ep.split_stride + (size_t)row * ep.ldo + col] = y;
         } else {  // MLP_EPI_DGELU
             float g = y * gelu_grad(ep.pre[(size_t)row * ep.ldo + col]);
             o16[v] = f2bf(g);
@@ -154,9 +155,48 @@ __global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict_
             }
             __syncthreads();
             transposed_tile_out<256>(T, (float *)(smem + 64 * TP), ep.out16T, ep.ldT, n0, m0,
-                                ep.cpart ? ep.cpart + (size_t)blockIdx.y * ep.ldo : nullptr);
+                                ep.cpart ? ep.cpart + (size_t)ty * ep.ldo : nullptr);
         }
     }
+}
+
+// grid (N / 64, M / 64, S): block (x, y, z) is tile (y, x) of slice z, k in
+// [z K / S, (z + 1) K / S)
+template <int EPI>
+__global__ __launch_bounds__(256) void mlp_gemm_kernel(const __bf16 *__restrict__ A, int lda,
+                                                       const __bf16 *__restrict__ B, int ldb, int K,
+                                                       MlpEpi ep)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 smem[GEMM_SMEM];
+    const int kspan = K / (int)gridDim.z;
+    mlp_gemm_tile<EPI>(A, lda, B, ldb, (int)blockIdx.z * kspan, kspan, (int)blockIdx.y * MT, (int)blockIdx.x * NT,
+                       (int)blockIdx.z, (int)blockIdx.y, ep, smem);
+}
+
+// The weight gradients of every layer in one launch: each layer is a
+// segment of the table (blocks from first on), its blocks ordered as
+// mlp_gemm_kernel's grid would be (x fastest, then y, then the slice).  Slice
+// z of a layer takes k in [z K / S, (z + 1) K / S) and writes its partial at
+// part + z M N; mlp_reduce_kernel adds the slices in order.
+__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const MlpWGradSeg *__restrict__ segs, int nseg, int K, int S,
+                                                        int ld)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 smem[GEMM_SMEM];
+    int k = 0;
+    for (int j = 1; j < nseg; ++j)
+        if ((int)blockIdx.x >= segs[j].first) k = j;
+    const MlpWGradSeg g = segs[k];
+    const int ntn = g.N / NT, ntm = g.M / MT;
+    int b = (int)blockIdx.x - g.first;
+    const int x = b % ntn;
+    b /= ntn;
+    const int y = b % ntm, z = b / ntm;
+    MlpEpi ep{};
+    ep.out32 = g.part;
+    ep.ldo = g.N;
+    ep.split_stride = (size_t)g.M * g.N;
+    const int kspan = K / S;
+    mlp_gemm_tile<MLP_EPI_F32>(g.A, ld, g.B, ld, z * kspan, kspan, y * MT, x * NT, z, y, ep, smem);
 }
 
 // output-layer gradient of the weighted squared error:
@@ -208,7 +248,7 @@ __global__ __launch_bounds__(TILE_THREADS) void mse_grad_kernel(const float *__r
 // db[n] = sum_m dY[m, n] (FP32 from the bf16 gradient), in two passes.  Pass
 // 1 is the epilogue of dY's producer (mse_grad_kernel, the backward-data GEMM):
 // per 64-row slab y and column n, part[y][n] adds the slab's four 16-row groups
-// in order, each summed row by row.  Pass 2 (colsum_total_kernel) adds the
+// in order, each summed row by row.  Pass 2 (mlp_reduce_kernel) adds the
 // slabs in order, so the result does not depend on scheduling.
 constexpr int CS_ROWS = 64;              // rows per slab
 
@@ -230,19 +270,18 @@ __device__ inline float ordered_strided_sum(const float *__restrict__ p, int n, 
     return t;
 }
 
-__global__ void colsum_total_kernel(const float *__restrict__ part, int slabs, int cols, float *__restrict__ db)
+// Several ordered sums in one launch (the split-k and column sums of every
+// layer): segment k owns the blocks from first_k on, dst[i] = src[i] +
+// src[stride + i] + ... (count terms, in order) for i < n.
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const MlpReduceSeg *__restrict__ segs, int nseg)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    db[c] = ordered_strided_sum(part + c, slabs, (size_t)cols);
-}
-
-// out[i] = sum_z part[z stride + i], z in order (split-k partial products)
-__global__ void splitk_sum_kernel(const float *__restrict__ part, int S, size_t stride, long n, float *__restrict__ out)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = ordered_strided_sum(part + i, S, stride);
+    int k = 0;
+    for (int j = 1; j < nseg; ++j)
+        if ((int)blockIdx.x >= segs[j].first) k = j;
+    const MlpReduceSeg g = segs[k];
+    const long i = (long)((int)blockIdx.x - g.first) * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    g.dst[i] = ordered_strided_sum(g.src + i, g.count, g.stride);
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master weights, every tensor of the
@@ -338,26 +377,16 @@ hipError_t launch_mse_grad(const float *yhat, const float *y, const float *w, in
 
 int colsum_slabs(int rows) { return (rows + CS_ROWS - 1) / CS_ROWS; }
 
-hipError_t launch_colsum_total(const float *part, int rows, int cols, float *db, hipStream_t st)
+hipError_t launch_mlp_wgrad(const MlpWGradSeg *segs_dev, int nseg, int blocks, int K, int S, int ld, hipStream_t st)
 {
-    hipLaunchKernelGGL(colsum_total_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, part, colsum_slabs(rows), cols,
-                       db);
+    if (S < 1 || K % (S * KT) || ld < K) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(blocks), dim3(256), 0, st, segs_dev, nseg, K, S, ld);
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_gemm_splitk(const void *A, int lda, const void *B, int ldb, int M, int N, int K, int S,
-                                  float *part, float *out, hipStream_t st)
+hipError_t launch_mlp_reduce(const MlpReduceSeg *segs_dev, int nseg, int blocks, hipStream_t st)
 {
-    MlpEpi ep{};
-    ep.out32 = part;
-    ep.ldo = N;
-    ep.split_stride = (size_t)M * N;
-    dim3 grid(N / NT, M / MT, S), block(256);
-    hipLaunchKernelGGL((mlp_gemm_kernel<MLP_EPI_F32>), grid, block, 0, st, (const __bf16 *)A, lda, (const __bf16 *)B,
-                       ldb, K, ep);
-    const long n = (long)M * N;
-    hipLaunchKernelGGL(splitk_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, S,
-                       ep.split_stride, n, out);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3(blocks), dim3(256), 0, st, segs_dev, nseg);
     return hipGetLastError();
 }
 
