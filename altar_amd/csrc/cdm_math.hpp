@@ -28,8 +28,8 @@ namespace altar {
 namespace cdm {
 
 // Arithmetic helpers.  In FP32 on the device they map to the hardware
-// reciprocal, square root and base-2 logarithm (about 1 ulp each); FP64 and
-// host builds use the correctly rounded operations.
+// reciprocal, square root, reciprocal square root and base-2 logarithm
+// (about 1 ulp each); host builds use the correctly rounded operations.
 ALTAR_HD inline float f_rcp(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -88,14 +88,6 @@ ALTAR_HD inline double f_rsq(double x)
     return 1.0 / sqrt(x);
 #endif
 }
-ALTAR_HD inline float f_log(float x)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
-#else
-    return logf(x);
-#endif
-}
 // FP64 natural logarithm.  On the device: x = m 2^e with m in [sqrt(1/2),
 // sqrt(2)), log m = 2 atanh(s) = 2 (s + s^3/3 + ... + s^19/19) with
 // s = (m - 1) / (m + 1), |s| <= 0.1716, so the omitted terms stay below
@@ -140,6 +132,20 @@ ALTAR_HD inline double f_log(double x)
     return log(x);
 #endif
 }
+
+// The logarithm of the side's log(r+a) and log(r-z3) differences, in the
+// unit its cL and cZ coefficients assume: the FP32 geometry carries them
+// multiplied by ln 2 (convert_side), so FP32 sides take the base-2 logarithm
+// (one hardware instruction, no scaling); FP64 sides the natural one.
+ALTAR_HD inline float side_log(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
+ALTAR_HD inline double side_log(double x) { return f_log(x); }
 
 // atan2 in FP32: octant reduction to t in [0, 1] and atan(t) = t + t^3 q(t^2)
 // with a degree-6 q from a minimax fit on [0, 1] (Lawson's iteratively
@@ -495,8 +501,8 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
                    v1B, v2B, v3B, fyB, fxB, rpaB);
     const T dFi = side_angle(fyB * fxA - fyA * fxB, fxA * fxB + fyA * fyB, exact_angle);
     // log((rB + aB) / (rA + aA)) and log((rB - z3B) / (rA - z3A))
-    const T dLr = f_log(rpaB * irpaA);
-    const T dLz = sg * f_log(rz);
+    const T dLr = side_log(rpaB * irpaA);
+    const T dLz = sg * side_log(rz);
     const T v1 = (v1B - v1A) + sd.cF[0] * dFi + sd.cL[0] * dLr + sd.cZ[0] * dLz;
     const T v2 = (v2B - v2A) + sd.cF[1] * dFi + sd.cL[1] * dLr + sd.cZ[1] * dLz;
     const T v3 = (v3B - v3A) + sd.cF[2] * dFi + sd.cL[2] * dLr + sd.cZ[2] * dLz;
@@ -505,9 +511,13 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
     uv = -v3;
 }
 
+// FP64 side constants to another precision.  An FP32 target takes cL and cZ
+// multiplied by ln 2 (in FP64, then rounded once), for side_log's base-2
+// logarithm.
 template <typename T, typename S>
 ALTAR_HD inline void convert_side(const Side<S> &a, Side<T> &b)
 {
+    const S lscale = sizeof(T) == 4 ? S(0.693147180559945309417232121458) : S(1);
     b.e1x = (T)a.e1x; b.e1y = (T)a.e1y;
     b.y1AB = (T)a.y1AB; b.y2AB = (T)a.y2AB;
     b.b1 = (T)a.b1; b.b2 = (T)a.b2; b.b3 = (T)a.b3;
@@ -516,8 +526,8 @@ ALTAR_HD inline void convert_side(const Side<S> &a, Side<T> &b)
     b.kt = (T)a.kt; b.e13 = (T)a.e13;
     for (int j = 0; j < 3; ++j) {
         b.cF[j] = (T)a.cF[j];
-        b.cL[j] = (T)a.cL[j];
-        b.cZ[j] = (T)a.cZ[j];
+        b.cL[j] = (T)(a.cL[j] * lscale);
+        b.cZ[j] = (T)(a.cZ[j] * lscale);
     }
 }
 
@@ -530,21 +540,7 @@ ALTAR_HD inline void convert(const Geom<S> &in, Geom<T> &out)
         out.va[v] = (T)in.va[v];
         out.ka2[v] = (T)in.ka2[v];
     }
-    for (int s = 0; s < 12; ++s) {
-        const Side<S> &a = in.side[s];
-        Side<T> &b = out.side[s];
-        b.e1x = (T)a.e1x; b.e1y = (T)a.e1y;
-        b.y1AB = (T)a.y1AB; b.y2AB = (T)a.y2AB;
-        b.b1 = (T)a.b1; b.b2 = (T)a.b2; b.b3 = (T)a.b3;
-        b.s = (T)a.s; b.c = (T)a.c; b.t = (T)a.t;
-        b.hI = (T)a.hI; b.hII = (T)a.hII;
-        b.kt = (T)a.kt; b.e13 = (T)a.e13;
-        for (int j = 0; j < 3; ++j) {
-            b.cF[j] = (T)a.cF[j];
-            b.cL[j] = (T)a.cL[j];
-            b.cZ[j] = (T)a.cZ[j];
-        }
-    }
+    for (int s = 0; s < 12; ++s) convert_side(in.side[s], out.side[s]);
 }
 
 // Full displacement at one point from a prepared geometry (host reference

