@@ -191,11 +191,13 @@ ALTAR_HD inline double side_angle(double y, double x, bool exact) { return exact
 template <typename T>
 struct Side {
     T e1x, e1y;          // first ADCS axis (horizontal); ey2 = (e1y, -e1x, 0), ey3 = -ez
-    T y1AB, y2AB;        // ADCS coordinates of PB - PA (libcdm.py:337-339)
+    T y1AB;              // ADCS y1 of PB - PA (libcdm.py:337-339); its y2 is zero, since ey2 is
+                         // normal to the side's horizontal direction, so both vertices share y2
+    T dz1, dz3;          // z1B - z1A and z3B - z3A (configuration II; configuration I flips both)
     T b1, b2, b3;        // Burgers vector in ADCS, divided by 2 pi
-    T s, c, t;           // sin, cos, cot of beta (configuration II)
+    T s, c;              // sin and cos of beta (configuration II)
     T hI, hII;           // cot of half the angle, configurations I and II
-    T kt, e13;           // k cot(beta) and b3 sin(beta) - b1 cos(beta)
+    T kt, e13, b3s;      // k cot(beta), b3 sin(beta) - b1 cos(beta) and b3 sin(beta)
     T cF[3], cL[3], cZ[3];
 };
 
@@ -223,42 +225,40 @@ ALTAR_HD constexpr int side_vB(int s) { return (s / 4) * 4 + ((s % 4) + 1) % 4; 
 //        + b3 s y2 q (r c + a)
 // (k = 1 - 2 nu; s, c the configuration's sine and cosine).
 //
-// vertex_pq returns z1 and r - z3 as the fraction p / qd (qd = 1 when
-// z3 <= 0; for z3 > 0, r - z3 = (z1^2 + y2^2) / (r + z3) without
-// cancellation).  vertex_rest takes irmz = 1 / (r - z3), formed by the
-// caller (one reciprocal serves both vertices of a side), and returns
-// v1, v2, v3 and the atan2 arguments of Fi.
+// vertex_p returns r - z3 as the fraction p / qd (qd = 1 when z3 <= 0; for
+// z3 > 0, r - z3 = (z1^2 + y2^2) / (r + z3) without cancellation).
+// vertex_rest takes irmz = 1 / (r - z3), formed by the caller (one
+// reciprocal serves both vertices of a side), and returns v1, v2, v3 and the
+// atan2 abscissa of Fi.  The two vertices of a side share y2 and with it
+// y2^2, y2 (b3 s - b1 c) and G3 = b3 s y2, which the caller forms once.
 template <typename T>
-ALTAR_HD inline void vertex_pq(T y1, T y2, T a, T r, T s, T c, T &z1, T &p, T &qd)
+ALTAR_HD inline void vertex_p(T z1, T z3, T y2sq, T r, T &p, T &qd)
 {
-    z1 = y1 * c + a * s;
-    const T z3 = y1 * s - a * c;
     const bool pos = z3 > T(0);
-    p = pos ? z1 * z1 + y2 * y2 : r - z3;
+    p = pos ? z1 * z1 + y2sq : r - z3;
     qd = pos ? r + z3 : T(1);
 }
 
 template <typename T>
-ALTAR_HD inline void vertex_rest(T y1, T y2, T a, T ka2, T r, T ir, T irpa, T s, T c, T h, T e13,
-                                 const Side<T> &sd, T k05, T nu2, T z1, T irmz, T &v1, T &v2, T &v3,
-                                 T &fy, T &fx, T &rpa)
+ALTAR_HD inline void vertex_rest(T y1, T y2, T y2sq, T y2e13, T G3, T a, T ka2, T r, T ir, T irpa, T s,
+                                 T c, T h, const Side<T> &sd, T k05, T nu2, T z1, T irmz, T &v1, T &v2,
+                                 T &v3, T &fx, T &rpa)
 {
     const T q = ir * irmz;
     const T u = r * s - y1;
     const T Y2I = y2 * irpa, Y1I = y1 * irpa;
     const T H = (sd.kt + k05 * Y1I) - y1 * ir;
-    const T W1 = y2 * e13 + sd.b2 * z1;
+    const T W1 = y2e13 + sd.b2 * z1;
     const T P1 = Y2I * H;
     const T t1 = y1 * H + ka2;
     v1 = sd.b1 * P1 - (sd.b2 * irpa) * t1 + (u * q) * W1;
-    const T inner2 = (sd.kt * y1 - ka2) + y2 * (y2 * ir - k05 * Y2I);
+    // y2 (y2 / r - k y2 / (2 (r + a))) = y2^2 (1 / r - k / (2 (r + a)))
+    const T inner2 = (sd.kt * y1 - ka2) + y2sq * (ir - k05 * irpa);
     v2 = -(sd.b1 * irpa) * inner2 - sd.b2 * P1 - (y2 * q) * W1;
     const T gg = nu2 + a * ir;
     const T ca = c + a * ir;
-    const T G3 = (sd.b3 * s) * y2;
     v3 = gg * (sd.b1 * Y2I - sd.b2 * Y1I) + (irmz * ca) * (W1 - G3) + (G3 * q) * (r * c + a);
     rpa = r + a;
-    fy = y2;
     fx = rpa * h - y1;
 }
 
@@ -281,7 +281,6 @@ ALTAR_HD inline bool make_side(const double PA[3], const double PB[3], const dou
     sd.e1x = e1x;
     sd.e1y = e1y;
     sd.y1AB = e1x * sv[0] + e1y * sv[1];
-    sd.y2AB = e1y * sv[0] - e1x * sv[1];
     // ey2 = ey3 x ey1 = (e1y, -e1x, 0); ey3 = (0, 0, -1)
     double b1 = e1x * bEF[0] + e1y * bEF[1];
     double b2 = e1y * bEF[0] - e1x * bEF[1];
@@ -295,12 +294,15 @@ ALTAR_HD inline bool make_side(const double PA[3], const double PB[3], const dou
     const double sB = sin(beta), cB = cos(beta), tB = 1.0 / tan(beta);
     sd.s = sB;
     sd.c = cB;
-    sd.t = tB;
+    // from A to B: y1 changes by -y1AB and the depth a = -z by -sv[2]
+    sd.dz1 = -sd.y1AB * cB - sv[2] * sB;
+    sd.dz3 = -sd.y1AB * sB + sv[2] * cB;
     sd.hII = 1.0 / tan(beta / 2.0);
     sd.hI = 1.0 / tan((-pi + beta) / 2.0);
     sd.kt = k * tB;
     double B1 = sd.b1, B2 = sd.b2, B3 = sd.b3;
     sd.e13 = B3 * sB - B1 * cB;
+    sd.b3s = B3 * sB;
     // Fi enters as 2 * atan2(...): the factor 2 is folded in here
     sd.cF[0] = 2.0 * (B1 * (1.0 - k * tB * tB));
     sd.cF[1] = 2.0 * (B2 * (1.0 + k * tB * tB));
@@ -455,11 +457,13 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
 
 // Displacement (ue, un, uv) of one side at a point, everything in type T.
 // dxA = X - xA etc.; rA, irA, irpaA are the per-vertex quantities (r, 1/r,
-// 1/(r+a)).  The second vertex's ADCS coordinates are formed from the first
-// one's as in libcdm.py:337-339, so both share the sign of y2 on the Burgers
-// function's branch cut.  With y2 of one sign at both vertices, both Burgers
-// angles lie in the same half plane and their difference is one atan2:
-// atan2(yB xA - yA xB, xA xB + yA yB).
+// 1/(r+a)).  The second vertex's ADCS coordinates follow from the first
+// one's (libcdm.py:335-337): y1B = y1A - y1AB, and y2B = y2A, because the
+// ADCS y2 axis is normal to the side's horizontal direction (the reference
+// subtracts the rotated side vector's y2, which is rounding, about 1e-13 m);
+// z1 and z3 at B are those at A plus side constants.  With one y2 at both
+// vertices, both Burgers angles lie in the same half plane and their
+// difference is one atan2: atan2(y2 (xA - xB), xA xB + y2^2).
 //
 // One reciprocal per side: D = 1 / (pA pB qB) gives 1 / (rA - z3A) =
 // qA pB qB D, 1 / (rB - z3B) = qB pA qB D and the log argument
@@ -476,30 +480,33 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
 {
     const T e1x = sd.e1x, e1y = sd.e1y;
     const T y1A = e1x * dxA + e1y * dyA;
-    const T y2A = e1y * dxA - e1x * dyA;
+    const T y2 = e1y * dxA - e1x * dyA;
     const T y1B = y1A - sd.y1AB;
-    const T y2B = y2A - sd.y2AB;
     // configuration I (y1A >= 0) uses beta - pi: sin, cos, cZ, e13 change sign
     const bool cfgI = y1A >= T(0);
     const T sg = cfgI ? T(-1) : T(1);
-    const T s = sg * sd.s, c = sg * sd.c, e13 = sg * sd.e13;
+    const T s = sg * sd.s, c = sg * sd.c;
     const T h = cfgI ? sd.hI : sd.hII;
     const T k05 = T(0.5) * k, nu2 = T(2) * nu;
-    T z1A, pA, qA, z1B, pB, qB;
-    vertex_pq<T>(y1A, y2A, aA, rA, s, c, z1A, pA, qA);
-    vertex_pq<T>(y1B, y2B, aB, rB, s, c, z1B, pB, qB);
+    const T y2sq = y2 * y2, y2e13 = (sg * sd.e13) * y2, G3 = (sg * sd.b3s) * y2;
+    const T z1A = y1A * c + aA * s, z3A = y1A * s - aA * c;
+    const T z1B = z1A + sg * sd.dz1, z3B = z3A + sg * sd.dz3;
+    T pA, qA, pB, qB;
+    vertex_p<T>(z1A, z3A, y2sq, rA, pA, qA);
+    vertex_p<T>(z1B, z3B, y2sq, rB, pB, qB);
     const T D = f_rcp((pA * pB) * qB);
     const T u1 = pB * D;                        // 1 / (pA qB)
     const T irmzA = qA * (qB * u1);
     const T irmzB = qB * (qB * (pA * D));
     const T rz = (pB * qA) * u1;                // (rB - z3B) / (rA - z3A)
-    T v1A, v2A, v3A, fyA, fxA, rpaA;
-    T v1B, v2B, v3B, fyB, fxB, rpaB;
-    vertex_rest<T>(y1A, y2A, aA, kaA, rA, irA, irpaA, s, c, h, e13, sd, k05, nu2, z1A, irmzA,
-                   v1A, v2A, v3A, fyA, fxA, rpaA);
-    vertex_rest<T>(y1B, y2B, aB, kaB, rB, irB, irpaB, s, c, h, e13, sd, k05, nu2, z1B, irmzB,
-                   v1B, v2B, v3B, fyB, fxB, rpaB);
-    const T dFi = side_angle(fyB * fxA - fyA * fxB, fxA * fxB + fyA * fyB, exact_angle);
+    T v1A, v2A, v3A, fxA, rpaA;
+    T v1B, v2B, v3B, fxB, rpaB;
+    vertex_rest<T>(y1A, y2, y2sq, y2e13, G3, aA, kaA, rA, irA, irpaA, s, c, h, sd, k05, nu2, z1A, irmzA,
+                   v1A, v2A, v3A, fxA, rpaA);
+    vertex_rest<T>(y1B, y2, y2sq, y2e13, G3, aB, kaB, rB, irB, irpaB, s, c, h, sd, k05, nu2, z1B, irmzB,
+                   v1B, v2B, v3B, fxB, rpaB);
+    // the angle from (fxA, y2) to (fxB, y2)
+    const T dFi = side_angle(y2 * (fxA - fxB), fxA * fxB + y2sq, exact_angle);
     // log((rB + aB) / (rA + aA)) and log((rB - z3B) / (rA - z3A))
     const T dLr = side_log(rpaB * irpaA);
     const T dLz = sg * side_log(rz);
@@ -519,11 +526,12 @@ ALTAR_HD inline void convert_side(const Side<S> &a, Side<T> &b)
 {
     const S lscale = sizeof(T) == 4 ? S(0.693147180559945309417232121458) : S(1);
     b.e1x = (T)a.e1x; b.e1y = (T)a.e1y;
-    b.y1AB = (T)a.y1AB; b.y2AB = (T)a.y2AB;
+    b.y1AB = (T)a.y1AB;
+    b.dz1 = (T)a.dz1; b.dz3 = (T)a.dz3;
     b.b1 = (T)a.b1; b.b2 = (T)a.b2; b.b3 = (T)a.b3;
-    b.s = (T)a.s; b.c = (T)a.c; b.t = (T)a.t;
+    b.s = (T)a.s; b.c = (T)a.c;
     b.hI = (T)a.hI; b.hII = (T)a.hII;
-    b.kt = (T)a.kt; b.e13 = (T)a.e13;
+    b.kt = (T)a.kt; b.e13 = (T)a.e13; b.b3s = (T)a.b3s;
     for (int j = 0; j < 3; ++j) {
         b.cF[j] = (T)a.cF[j];
         b.cL[j] = (T)(a.cL[j] * lscale);
