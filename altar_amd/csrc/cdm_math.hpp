@@ -198,7 +198,8 @@ struct Side {
     T s, c;              // sin and cos of beta (configuration II)
     T hI, hII;           // cot of half the angle, configurations I and II
     T kt, e13, b3s;      // k cot(beta), b3 sin(beta) - b1 cos(beta) and b3 sin(beta)
-    T cF[3], cL[3], cZ[3];
+    T cF[3], cL[3], cZ[3];   // coefficients of dFi/2, dlog(r+a), dlog(r-z3) in the east, north and
+                             // up components (the ADCS ones rotated to the EFCS)
 };
 
 // per-chain geometry: 12 vertices (x, y, depth a = -z) and 12 sides
@@ -244,20 +245,24 @@ ALTAR_HD inline void vertex_rest(T y1, T y2, T y2sq, T y2e13, T G3, T a, T ka2, 
                                  T c, T h, const Side<T> &sd, T k05, T nu2, T z1, T irmz, T &v1, T &v2,
                                  T &v3, T &fx, T &rpa)
 {
+    // the formulas above with B1 = b1/(r+a), B2 = b2/(r+a), qW = q W1 and
+    // K = k/(2 (r+a)) - 1/r, so that H = k t + y1 K and the bracket of v2 is
+    // k y1 t - k a/2 - y2^2 K
     const T q = ir * irmz;
     const T u = r * s - y1;
-    const T Y2I = y2 * irpa, Y1I = y1 * irpa;
-    const T H = (sd.kt + k05 * Y1I) - y1 * ir;
+    const T B1 = sd.b1 * irpa, B2 = sd.b2 * irpa;
+    const T K = k05 * irpa - ir;
+    const T H = sd.kt + y1 * K;
     const T W1 = y2e13 + sd.b2 * z1;
-    const T P1 = Y2I * H;
+    const T yH = y2 * H;
     const T t1 = y1 * H + ka2;
-    v1 = sd.b1 * P1 - (sd.b2 * irpa) * t1 + (u * q) * W1;
-    // y2 (y2 / r - k y2 / (2 (r + a))) = y2^2 (1 / r - k / (2 (r + a)))
-    const T inner2 = (sd.kt * y1 - ka2) + y2sq * (ir - k05 * irpa);
-    v2 = -(sd.b1 * irpa) * inner2 - sd.b2 * P1 - (y2 * q) * W1;
+    const T qW = q * W1;
+    v1 = B1 * yH - B2 * t1 + u * qW;
+    const T inner2 = (sd.kt * y1 - ka2) - y2sq * K;
+    v2 = -(B1 * inner2 + B2 * yH + y2 * qW);
     const T gg = nu2 + a * ir;
     const T ca = c + a * ir;
-    v3 = gg * (sd.b1 * Y2I - sd.b2 * Y1I) + (irmz * ca) * (W1 - G3) + (G3 * q) * (r * c + a);
+    v3 = gg * (B1 * y2 - B2 * y1) + (irmz * ca) * (W1 - G3) + (G3 * q) * (r * c + a);
     rpa = r + a;
     fx = rpa * h - y1;
 }
@@ -313,6 +318,14 @@ ALTAR_HD inline bool make_side(const double PA[3], const double PB[3], const dou
     sd.cZ[0] = -B2 * k * tB * tB * cB;
     sd.cZ[1] = -B1 * k * tB / sB;
     sd.cZ[2] = B2 * k * tB * cB;
+    // ADCS (v1, v2, v3) -> EFCS (e1x v1 + e1y v2, e1y v1 - e1x v2, -v3)
+    double *cs[3] = {sd.cF, sd.cL, sd.cZ};
+    for (double *cc : cs) {
+        const double c0 = cc[0], c1 = cc[1], c2 = cc[2];
+        cc[0] = e1x * c0 + e1y * c1;
+        cc[1] = e1y * c0 - e1x * c1;
+        cc[2] = -c2;
+    }
     return true;
 }
 
@@ -455,7 +468,8 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
     return flags;
 }
 
-// Displacement (ue, un, uv) of one side at a point, everything in type T.
+// Displacement (ue, un, uv) of one side at a point, everything in type T,
+// added to ue, un, uv.
 // dxA = X - xA etc.; rA, irA, irpaA are the per-vertex quantities (r, 1/r,
 // 1/(r+a)).  The second vertex's ADCS coordinates follow from the first
 // one's (libcdm.py:335-337): y1B = y1A - y1AB, and y2B = y2A, because the
@@ -510,12 +524,11 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
     // log((rB + aB) / (rA + aA)) and log((rB - z3B) / (rA - z3A))
     const T dLr = side_log(rpaB * irpaA);
     const T dLz = sg * side_log(rz);
-    const T v1 = (v1B - v1A) + sd.cF[0] * dFi + sd.cL[0] * dLr + sd.cZ[0] * dLz;
-    const T v2 = (v2B - v2A) + sd.cF[1] * dFi + sd.cL[1] * dLr + sd.cZ[1] * dLz;
-    const T v3 = (v3B - v3A) + sd.cF[2] * dFi + sd.cL[2] * dLr + sd.cZ[2] * dLz;
-    ue = e1x * v1 + e1y * v2;
-    un = e1y * v1 - e1x * v2;
-    uv = -v3;
+    // added to the running sums, one multiply-add per term
+    const T d1 = v1B - v1A, d2 = v2B - v2A, d3 = v3B - v3A;
+    ue = sd.cZ[0] * dLz + (sd.cL[0] * dLr + (sd.cF[0] * dFi + (e1y * d2 + (e1x * d1 + ue))));
+    un = sd.cZ[1] * dLz + (sd.cL[1] * dLr + (sd.cF[1] * dFi + ((un - e1x * d2) + e1y * d1)));
+    uv = sd.cZ[2] * dLz + (sd.cL[2] * dLr + (sd.cF[2] * dFi + (uv - d3)));
 }
 
 // FP64 side constants to another precision.  An FP32 target takes cL and cZ
@@ -577,15 +590,11 @@ inline void point_disp(const Geom<T> &G, const Geom<double> &G64, unsigned flags
             double dxb = X - G64.vx[b], dyb = Y - G64.vy[b], ab = G64.va[b];
             double ra = sqrt(dxa * dxa + dya * dya + aa * aa);
             double rb = sqrt(dxb * dxb + dyb * dyb + ab * ab);
-            double e, n, u;
             side_disp<double>(G64.side[s], nu, 1.0 - 2.0 * nu, dxa, dya, aa, G64.ka2[a], ra, 1.0 / ra,
-                              1.0 / (ra + aa), ab, G64.ka2[b], rb, 1.0 / rb, 1.0 / (rb + ab), e, n, u);
-            due += e; dun += n; duv += u;
+                              1.0 / (ra + aa), ab, G64.ka2[b], rb, 1.0 / rb, 1.0 / (rb + ab), due, dun, duv);
         } else {
-            T e, n, u;
             side_disp<T>(G.side[s], tnu, tk, dx[a], dy[a], G.va[a], G.ka2[a], r[a], ir[a], irpa[a],
-                         G.va[b], G.ka2[b], r[b], ir[b], irpa[b], e, n, u);
-            sue += e; sun += n; suv += u;
+                         G.va[b], G.ka2[b], r[b], ir[b], irpa[b], sue, sun, suv);
         }
     }
     ue = (double)sue + due;

@@ -239,16 +239,12 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                         const double rb2 = dxb * dxb + dyb * dyb + ab * ab;
                         const double ira = cdm::f_rsq(ra2), irb = cdm::f_rsq(rb2);
                         const double ra = ra2 * ira, rb = rb2 * irb;
-                        double e, nn, u;
                         cdm::side_disp<double>(H.side[s], (double)nu, 1.0 - 2.0 * (double)nu, dxa, dya, aa,
                                                H.ka2[a], ra, ira, cdm::f_rcp(ra + aa), ab, H.ka2[b], rb,
-                                               irb, cdm::f_rcp(rb + ab), e, nn, u, exact64 != 0);
-                        due += e; dun += nn; duv += u;
+                                               irb, cdm::f_rcp(rb + ab), due, dun, duv, exact64 != 0);
                     } else {
-                        float e, nn, u;
                         cdm::side_disp<float>(G.side[s], nu, k, dx[ja], dy[ja], G.va[a], G.ka2[a], r[ja], ir[ja],
-                                              irpa[ja], G.va[b], G.ka2[b], r[jb], ir[jb], irpa[jb], e, nn, u);
-                        ue += e; un += nn; uv += u;
+                                              irpa[ja], G.va[b], G.ka2[b], r[jb], ir[jb], irpa[jb], ue, un, uv);
                     }
                 }
             }
@@ -405,10 +401,8 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const c
             const double rb2 = dxb * dxb + dyb * dyb + ab * ab;
             const double ira = cdm::f_rsq(ra2), irb = cdm::f_rsq(rb2);
             const double ra = ra2 * ira, rb = rb2 * irb;
-            double e, nn, u;
             cdm::side_disp<double>(H.side[s], nu, k, dxa, dya, aa, H.ka2[a], ra, ira, cdm::f_rcp(ra + aa),
-                                   ab, H.ka2[b], rb, irb, cdm::f_rcp(rb + ab), e, nn, u, EXACT);
-            due += e; dun += nn; duv += u;
+                                   ab, H.ka2[b], rb, irb, cdm::f_rcp(rb + ab), due, dun, duv, EXACT);
         }
         if (n < obs.N) {
             const double u = due * lx + dun * ly + duv * lz;
