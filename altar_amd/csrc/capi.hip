@@ -129,8 +129,7 @@ struct altar_source {
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
-    DevBuf R32, Rhi, Rlo, R64, Y64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census, fp64_list,
-        fp64_count;
+    DevBuf R32, Rhi, Rlo, R64, Y64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census;
     // rocBLAS handle of the FP64 whitening (dtrmm), created on first use
     rocblas_handle blas = nullptr;
     // optional per-kernel timing with HIP events on the launch stream
@@ -162,7 +161,6 @@ struct altar_source {
         R32.release(); Rhi.release(); Rlo.release(); R64.release(); Y64.release(); T32.release(); Thi.release();
         Tlo.release();
         partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release(); census.release();
-        fp64_list.release(); fp64_count.release();
     }
     size_t ntiles_obs() const { return (N + 255) / 256; }
 };
@@ -213,8 +211,6 @@ int altar_source_destroy(altar_source *h)
                 (void)hipEventDestroy(p.second);
             }
         h->census.release();
-        h->fp64_list.release();
-        h->fp64_count.release();
         h->Y64.release();
         if (h->blas) (void)rocblas_destroy_handle(h->blas);
         for (WBlock &b : h->blocks) {
@@ -471,10 +467,6 @@ CdmArgs cdm_args(altar_source *h, const double *theta, int C, bool fwd)
     a.g64 = h->g64.as<cdm::Geom<double>>();
     a.flags = h->flags.as<unsigned>();
     a.shifts = h->shifts.as<float>();
-    h->fp64_list.reserve((size_t)h->Ccap * sizeof(int));
-    h->fp64_count.reserve(sizeof(int));
-    a.fp64_list = h->fp64_list.as<int>();
-    a.fp64_count = h->fp64_count.as<int>();
     a.obs = h->obs(fwd);
     a.out = Out{};
     return a;

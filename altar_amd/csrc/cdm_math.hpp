@@ -206,6 +206,7 @@ struct Side {
 template <typename T>
 struct Geom {
     T vx[12], vy[12], va[12];
+    T va2[12];           // a^2 per vertex
     T ka2[12];           // (1 - 2 nu) a / 2 per vertex
     Side<T> side[12];
 };
@@ -236,8 +237,9 @@ template <typename T>
 ALTAR_HD inline void vertex_p(T z1, T z3, T y2sq, T r, T &p, T &qd)
 {
     const bool pos = z3 > T(0);
-    p = pos ? z1 * z1 + y2sq : r - z3;
-    qd = pos ? r + z3 : T(1);
+    const T rw = r + fabs(z3);   // r - z3 when z3 <= 0, r + z3 when z3 > 0
+    p = pos ? z1 * z1 + y2sq : rw;
+    qd = pos ? rw : T(1);
 }
 
 template <typename T>
@@ -462,6 +464,7 @@ ALTAR_HD inline unsigned make_geometry(double X0, double Y0, double depth, doubl
         G.vx[v] = V[v][0];
         G.vy[v] = V[v][1];
         G.va[v] = -V[v][2];
+        G.va2[v] = G.va[v] * G.va[v];
         G.ka2[v] = 0.5 * (1.0 - 2.0 * nu) * G.va[v];
     }
     for (int s = 0; s < 12; ++s) flags |= make_side_of(V, ax, ay, az, opening, nu, fp64_sin, s, G.side[s]);
@@ -502,7 +505,8 @@ ALTAR_HD inline void side_disp(const Side<T> &sd, T nu, T k,
     const T s = sg * sd.s, c = sg * sd.c;
     const T h = cfgI ? sd.hI : sd.hII;
     const T k05 = T(0.5) * k, nu2 = T(2) * nu;
-    const T y2sq = y2 * y2, y2e13 = (sg * sd.e13) * y2, G3 = (sg * sd.b3s) * y2;
+    const T sy2 = sg * y2;
+    const T y2sq = y2 * y2, y2e13 = sd.e13 * sy2, G3 = sd.b3s * sy2;
     const T z1A = y1A * c + aA * s, z3A = y1A * s - aA * c;
     const T z1B = z1A + sg * sd.dz1, z3B = z3A + sg * sd.dz3;
     T pA, qA, pB, qB;
@@ -559,6 +563,7 @@ ALTAR_HD inline void convert(const Geom<S> &in, Geom<T> &out)
         out.vx[v] = (T)in.vx[v];
         out.vy[v] = (T)in.vy[v];
         out.va[v] = (T)in.va[v];
+        out.va2[v] = (T)in.va2[v];
         out.ka2[v] = (T)in.ka2[v];
     }
     for (int s = 0; s < 12; ++s) convert_side(in.side[s], out.side[s]);

@@ -59,8 +59,6 @@ struct CdmArgs {
     cdm::Geom<double> *g64;
     unsigned *flags;
     float *shifts;        // C x MAX_SETS
-    int *fp64_list;       // C: chains with FP64 sides (residual modes), built on the device
-    int *fp64_count;      // 1
     ObsView obs;
     Out out;
 };

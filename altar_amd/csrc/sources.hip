@@ -152,6 +152,8 @@ __global__ __launch_bounds__(PREP_CHAINS * 12) void cdm_prepare_sides_kernel(
                 const double va = -V[v][2];
                 const double ka2 = 0.5 * (1.0 - 2.0 * nu) * va;
                 g64[c].vx[v] = V[v][0]; g64[c].vy[v] = V[v][1]; g64[c].va[v] = va; g64[c].ka2[v] = ka2;
+                g64[c].va2[v] = va * va;
+                g32[c].va2[v] = (float)(va * va);
                 g32[c].vx[v] = (float)V[v][0]; g32[c].vy[v] = (float)V[v][1];
                 g32[c].va[v] = (float)va; g32[c].ka2[v] = (float)ka2;
             }
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
                     dx[j] = X - G.vx[v];
                     dy[j] = Y - G.vy[v];
                     float a = G.va[v];
-                    const float r2 = dx[j] * dx[j] + dy[j] * dy[j] + a * a;
+                    const float r2 = dx[j] * dx[j] + (dy[j] * dy[j] + G.va2[v]);
                     ir[j] = cdm::f_rsq(r2);
                     r[j] = r2 * ir[j];
                     irpa[j] = cdm::f_rcp(r[j] + a);
@@ -324,63 +326,29 @@ hipError_t launch_cdm(const CdmArgs &a, int mode, hipStream_t st)
     return launch_cdm_main(a, mode, st);
 }
 
-// Chains that have FP64 sides (valid chains only), in chain order: one
-// workgroup, each thread a contiguous range, a fixed-order scan of the counts.
-constexpr int LIST_THREADS = 1024;
-__global__ __launch_bounds__(LIST_THREADS) void fp64_chain_list_kernel(const unsigned *__restrict__ flags, int C,
-                                                                       int *__restrict__ list,
-                                                                       int *__restrict__ count)
-{
-    __shared__ int part[LIST_THREADS];
-    const int per = (C + LIST_THREADS - 1) / LIST_THREADS;
-    const int b = threadIdx.x * per, e = min(C, b + per);
-    auto has64 = [&](int c) {
-        const unsigned f = flags[c];
-        return !(f & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) && (((f >> 12) & ~f & 0xfffu) != 0u);
-    };
-    int n = 0;
-    for (int c = b; c < e; ++c) n += has64(c) ? 1 : 0;
-    part[threadIdx.x] = n;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t < LIST_THREADS; ++t) {
-            const int v = part[t];
-            part[t] = run;
-            run += v;
-        }
-        *count = run;
-    }
-    __syncthreads();
-    int at = part[threadIdx.x];
-    for (int c = b; c < e; ++c)
-        if (has64(c)) list[at++] = c;
-}
-
-// The FP64 sides of the listed chains, added to the residual matrix (FP32
-// or FP64).  Block (tile, g) walks the list from g in steps of gridDim.y;
-// the loop ends for every block at the list's length.  EXACT: the Burgers
-// angle from the FP64 atan2 (every side in FP64, the reference's precision).
-// Each chain's FP64 geometry is copied into LDS, and the side loop is not
-// unrolled (a chain has two or four FP64 sides of twelve): 114 instead of 136
-// VGPRs, and 2% off the whole forward at 16,384 chains x 5,000 LOS.
+// The FP64 sides of the chains that have any, added to the residual matrix
+// (FP32 or FP64).  Block (tile, g) visits chains g, g + gridDim.y, ... and
+// skips those without FP64 sides (a wave-uniform test of the chain's flags);
+// the loop ends for every block at C.  EXACT: the Burgers angle from the
+// FP64 atan2 (every side in FP64, the reference's precision).  Each chain's
+// FP64 geometry is copied into LDS, and the side loop is not unrolled (a
+// chain has two or four FP64 sides of twelve): 114 instead of 136 VGPRs, and
+// 2% off the whole forward at 16,384 chains x 5,000 LOS.
 template <int MODE, bool EXACT>
 __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const cdm::Geom<double> *__restrict__ g64,
-                                                            const unsigned *__restrict__ flags,
-                                                            const int *__restrict__ list,
-                                                            const int *__restrict__ count, double nu, Out out)
+                                                            const unsigned *__restrict__ flags, int C, double nu,
+                                                            Out out)
 {
     const int n = blockIdx.x * OBS_TILE + threadIdx.x;
-    const int cnt = *count;
     double Xd = 0.0, Yd = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
     if (n < obs.N) {
         Xd = obs.xd[n]; Yd = obs.yd[n];
         lx = obs.lxd[n]; ly = obs.lyd[n]; lz = obs.lzd[n];
     }
     const double k = 1.0 - 2.0 * nu;
-    for (int i = blockIdx.y; i < cnt; i += gridDim.y) {
-        const int chain = list[i];
+    for (int chain = blockIdx.y; chain < C; chain += gridDim.y) {
         const unsigned fl = flags[chain];
+        if ((fl & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) || !((fl >> 12) & ~fl & 0xfffu)) continue;
         __shared__ cdm::Geom<double> sH;
         {
             constexpr int NV4 = (int)(sizeof(cdm::Geom<double>) / 16);
@@ -424,8 +392,7 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
     // residual modes take the two-pass path (FP32 sides, then the FP64 sides
     // of the chains that have any); the partial-sum and BF16 epilogues need
     // the complete residual in one kernel
-    const bool split = (mode == MODE_RES_F32 || mode == MODE_RES_F64 || mode == MODE_ACC_F32 ||
-                        mode == MODE_ACC_F64) && a.fp64_list != nullptr && a.fp64_count != nullptr;
+    const bool split = mode == MODE_RES_F32 || mode == MODE_RES_F64 || mode == MODE_ACC_F32 || mode == MODE_ACC_F64;
 #define CDM_LAUNCH(M, I) hipLaunchKernelGGL((cdm_kernel<M, I>), grid, block, 0, st, a.obs, a.g32, a.g64, a.flags, a.shifts, a.L.nsets, nu, exact64, a.out)
     if (!split) {
         if (mode == MODE_DIAG) CDM_LAUNCH(MODE_DIAG, true);
@@ -443,12 +410,8 @@ hipError_t launch_cdm_main(const CdmArgs &a, int mode, hipStream_t st)
 #undef CDM_LAUNCH
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fp64_chain_list_kernel, dim3(1), dim3(LIST_THREADS), 0, st, a.flags, a.C, a.fp64_list,
-                       a.fp64_count);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
     dim3 g2((a.obs.N + OBS_TILE - 1) / OBS_TILE, (unsigned)std::min(a.C, 2048));
-#define FP64_LAUNCH(M, E) hipLaunchKernelGGL((cdm_fp64_kernel<M, E>), g2, block, 0, st, a.obs, a.g64, a.flags, a.fp64_list, a.fp64_count, a.nu, a.out)
+#define FP64_LAUNCH(M, E) hipLaunchKernelGGL((cdm_fp64_kernel<M, E>), g2, block, 0, st, a.obs, a.g64, a.flags, a.C, a.nu, a.out)
     if (mode == MODE_RES_F32 || mode == MODE_ACC_F32) {
         if (exact64) FP64_LAUNCH(MODE_ACC_F32, true); else FP64_LAUNCH(MODE_ACC_F32, false);
     } else {

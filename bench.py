@@ -34,9 +34,9 @@ PEAK_HBM_GBS = 8000.0
 # residual, counted from the source expressions of altar_amd/csrc/cdm_math.hpp
 # and sources.hip: add, subtract, multiply = 1 each (a fused multiply-add = 2),
 # reciprocal, reciprocal square root, logarithm and atan2 = 1 each.  12 vertices
-# x 11 + 12 sides x 180 + 10 for the LOS projection and residual; 72 of them
+# x 10 + 12 sides x 177 + 10 for the LOS projection and residual; 72 of them
 # are reciprocals, reciprocal square roots, logarithms or atan2.
-CDM_FLOPS_PER_PAIR = 2302
+CDM_FLOPS_PER_PAIR = 2254
 
 # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) on the
 # default workload; lines "kernel <name> fetch_bytes <f> write_bytes <w>"

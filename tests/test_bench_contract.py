@@ -22,7 +22,7 @@ def test_traffic_summary_is_read():
     import bench
     t = bench.measured_traffic("cdm_forward")
     assert t is None or t > 0.3e9    # at least the residual matrix (16,384 x 5,120 FP32) is written
-    assert bench.CDM_FLOPS_PER_PAIR == 12 * 11 + 12 * 180 + 10
+    assert bench.CDM_FLOPS_PER_PAIR == 12 * 10 + 12 * 177 + 10
 
 
 def test_autocorrelation_times_on_ar1_chains():
