@@ -911,9 +911,12 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
             }
         }
         // one triangular GEMM: columns [off, off + npad) of R against W (npad x npad)
-        auto whiten = [&](size_t off, int npad, const DevBuf &W32, const DevBuf &Whi, const DevBuf &Wlo, int tile0) {
+        // nvalid: the block's observations; W's rows past them are zero
+        auto whiten = [&](size_t off, int npad, int nvalid, const DevBuf &W32, const DevBuf &Whi, const DevBuf &Wlo,
+                          int tile0) {
             GemmEpi e = ep;
             e.partial = ep.partial + tile0;
+            e.nvalid = nvalid;
             if (wr_out) e.out = wr_out + off;  // this block's columns of the whitened fields
             if (!bf16)
                 hip_check(launch_gemm_f32(h->R32.as<float>() + off, Np, W32.as<float>(), (size_t)npad, (int)Mp, npad,
@@ -928,11 +931,11 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         {
             Mark m(h, 1, st);
             if (!blocked) {
-                whiten(0, (int)Np, h->W32, h->Whi, h->Wlo, 0);
+                whiten(0, (int)Np, h->N, h->W32, h->Whi, h->Wlo, 0);
             } else {
                 int tile0 = 0;
                 for (const WBlock &B : h->blocks) {
-                    whiten((size_t)B.off, B.npad, B.W32, B.Whi, B.Wlo, tile0);
+                    whiten((size_t)B.off, B.npad, B.n, B.W32, B.Whi, B.Wlo, tile0);
                     tile0 += B.npad / 128;
                 }
             }

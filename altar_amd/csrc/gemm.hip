@@ -27,6 +27,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 namespace altar {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -188,35 +190,48 @@ __global__ __launch_bounds__(256) void gemm_nt_f32(const float *__restrict__ A, 
         sstore(0);
     }
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
-        // fragments: lane (half, l32) holds k = 8 half .. 8 half + 7 of row l32
-        floatx4 a[2][2], b[2][2];
+    // NI: how many of this wave's two 32-column MFMA tiles hold a column
+    // below ep.nvalid.  A tile of padding columns only would multiply zero
+    // rows of B; its MFMAs are skipped (its accumulator stays zero) while the
+    // wave still stages operands and meets every barrier.
+    auto kloop = [&](auto NI) {
+        for (int kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) gload(kt + 1);
+            // fragments: lane (half, l32) holds k = 8 half .. 8 half + 7 of row l32
+            floatx4 a[2][2], b[2][2];
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi) {
-            const float *p = &As[cur][(wm * 64 + mi * 32 + l32) * LDS32];
-            a[mi][0] = *(const floatx4 *)(p + c0);
-            a[mi][1] = *(const floatx4 *)(p + c1);
+            for (int mi = 0; mi < 2; ++mi) {
+                const float *p = &As[cur][(wm * 64 + mi * 32 + l32) * LDS32];
+                a[mi][0] = *(const floatx4 *)(p + c0);
+                a[mi][1] = *(const floatx4 *)(p + c1);
+            }
+#pragma unroll
+            for (int ni = 0; ni < decltype(NI)::value; ++ni) {
+                const float *p = &Bs[cur][(wn * 64 + ni * 32 + l32) * LDS32];
+                b[ni][0] = *(const floatx4 *)(p + c0);
+                b[ni][1] = *(const floatx4 *)(p + c1);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < decltype(NI)::value; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                            a[mi][kk >> 2][kk & 3], b[ni][kk >> 2][kk & 3], acc[mi][ni], 0, 0, 0);
+            }
+            if (kt + 1 < nk) sstore(cur ^ 1);
+            __syncthreads();
         }
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            const float *p = &Bs[cur][(wn * 64 + ni * 32 + l32) * LDS32];
-            b[ni][0] = *(const floatx4 *)(p + c0);
-            b[ni][1] = *(const floatx4 *)(p + c1);
-        }
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                        a[mi][kk >> 2][kk & 3], b[ni][kk >> 2][kk & 3], acc[mi][ni], 0, 0, 0);
-        }
-        if (kt + 1 < nk) sstore(cur ^ 1);
-        __syncthreads();
-    }
+    };
+    const int col0 = n0 + wn * 64;
+    if (ep.nvalid <= 0 || col0 + 32 < ep.nvalid)
+        kloop(std::integral_constant<int, 2>{});
+    else if (col0 < ep.nvalid)
+        kloop(std::integral_constant<int, 1>{});
+    else
+        kloop(std::integral_constant<int, 0>{});
     gemm_epilogue<EPI>(acc, ep, m0, n0, tn, wm, wn, red);
 }
 

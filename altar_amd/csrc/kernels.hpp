@@ -69,6 +69,8 @@ struct GemmEpi {
     size_t ldo;
     float *partial;       // SUMSQ: Mp x ntiles
     int ntiles;
+    int nvalid;           // columns from nvalid on are zero padding of B; their MFMAs are
+                          // skipped (0: every column is computed)
 };
 
 // surrogate MLP GEMM epilogues
