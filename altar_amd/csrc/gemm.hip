@@ -307,41 +307,51 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
         sstore(0);
     }
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
-        // lane (half, l32) takes k = 16 half .. 16 half + 15 of row l32 as two
-        // groups of 8 (one per MFMA)
+    // NI: as in gemm_nt_f32, the 32-column tiles of this wave below ep.nvalid
+    auto kloop = [&](auto NI) {
+        for (int kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) gload(kt + 1);
+            // lane (half, l32) takes k = 16 half .. 16 half + 15 of row l32 as two
+            // groups of 8 (one per MFMA)
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-            bf16x8 ah[2], bh[2], al[2], bl[2];
-            const int ck = 8 * ((2 * half + kk) ^ swz(l32));
+            for (int kk = 0; kk < 2; ++kk) {
+                bf16x8 ah[2], bh[2], al[2], bl[2];
+                const int ck = 8 * ((2 * half + kk) ^ swz(l32));
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi) {
-                const int off = (wm * 64 + mi * 32 + l32) * LDS16 + ck;
-                ah[mi] = *(const bf16x8 *)(&Ash[cur][off]);
-                if (!plain) al[mi] = *(const bf16x8 *)(&Asl[cur][off]);
-            }
-#pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
-                const int off = (wn * 64 + ni * 32 + l32) * LDS16 + ck;
-                bh[ni] = *(const bf16x8 *)(&Bsh[cur][off]);
-                if (!plain) bl[ni] = *(const bf16x8 *)(&Bsl[cur][off]);
-            }
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 2; ++ni) {
-                    if (!plain) {
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-                    }
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                for (int mi = 0; mi < 2; ++mi) {
+                    const int off = (wm * 64 + mi * 32 + l32) * LDS16 + ck;
+                    ah[mi] = *(const bf16x8 *)(&Ash[cur][off]);
+                    if (!plain) al[mi] = *(const bf16x8 *)(&Asl[cur][off]);
                 }
+#pragma unroll
+                for (int ni = 0; ni < decltype(NI)::value; ++ni) {
+                    const int off = (wn * 64 + ni * 32 + l32) * LDS16 + ck;
+                    bh[ni] = *(const bf16x8 *)(&Bsh[cur][off]);
+                    if (!plain) bl[ni] = *(const bf16x8 *)(&Bsl[cur][off]);
+                }
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < decltype(NI)::value; ++ni) {
+                        if (!plain) {
+                            acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                            acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+                        }
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                    }
+            }
+            if (kt + 1 < nk) sstore(cur ^ 1);
+            __syncthreads();
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
-        __syncthreads();
-    }
+    };
+    const int col0 = n0 + wn * 64;
+    if (ep.nvalid <= 0 || col0 + 32 < ep.nvalid)
+        kloop(std::integral_constant<int, 2>{});
+    else if (col0 < ep.nvalid)
+        kloop(std::integral_constant<int, 1>{});
+    else
+        kloop(std::integral_constant<int, 0>{});
     gemm_epilogue<EPI>(acc, ep, m0, n0, tn, wm, wn, red);
 }
 
