@@ -687,7 +687,10 @@ def mixing_and_moments(ann, da, plain, start, subs, times, ref_steps, ref_burn, 
       the worst parameter's.
     * Standard error of a run's mean of parameter k: sd_k sqrt(tau_k / (C L))
       over its L kept steps.  z_k = (mean - reference mean) / sqrt(se^2 +
-      se_ref^2); chi2 = sum_k z_k^2 (10 parameters: 18.3 is the 95% point)."""
+      se_ref^2); chi2 = sum_k z_k^2 (10 parameters: 18.3 is the 95% point).
+    * The same against plain Metropolis's means over the run's own kept
+      steps (vs_plain_same_window): every run starts from the same population,
+      so a slow drift of the population away from it enters both alike."""
     import torch
     sur = da.surrogate
     C, P = start.theta.shape
@@ -712,7 +715,7 @@ def mixing_and_moments(ann, da, plain, start, subs, times, ref_steps, ref_burn, 
         w = tr.window.double()
         runs[name] = {"steps": steps, "burn": burn, "wall_s": wall, "tau": tau, "tau_lower_bound": cut, "rho": rhos,
                       "mean": tr.means[burn:].mean(0).cpu().numpy(), "sd": w.reshape(-1, P).std(0).cpu().numpy(),
-                      "kept": steps - burn}
+                      "kept": steps - burn, "means": tr.means.cpu().numpy()}
 
     def run_plain(st, steps):
         ann.sampler = plain
@@ -768,6 +771,14 @@ def mixing_and_moments(ann, da, plain, start, subs, times, ref_steps, ref_burn, 
             o["vs_plain_reference"] = {"z": z.tolist(), "max_abs_z": float(np.abs(z).max()),
                                        "chi2": float((z ** 2).sum()),
                                        "sd_ratio": (r["sd"] / ref["sd"]).tolist()}
+            w0, w1 = r["burn"], r["steps"]
+            if w1 <= ref["steps"]:
+                ref_w = ref["means"][w0:w1].mean(0)
+                ref_se_w = ref["sd"] * np.sqrt(ref["tau"] / (C * (w1 - w0)))
+                zw = (r["mean"] - ref_w) / np.sqrt(se ** 2 + ref_se_w ** 2)
+                o["vs_plain_same_window"] = {"steps": [w0, w1], "z": zw.tolist(),
+                                             "max_abs_z": float(np.abs(zw).max()),
+                                             "chi2": float((zw ** 2).sum())}
         out[name] = o
     base = out["plain"]["effective_samples_per_s_min"]
     for name in ("da", "trajectory", "trajectory_biased_surrogate"):
