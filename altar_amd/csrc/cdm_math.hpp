@@ -91,7 +91,7 @@ ALTAR_HD inline double f_rsq(double x)
 // FP64 natural logarithm.  On the device: x = m 2^e with m in [sqrt(1/2),
 // sqrt(2)), log m = 2 atanh(s) = 2 (s + s^3/3 + ... + s^19/19) with
 // s = (m - 1) / (m + 1), |s| <= 0.1716, so the omitted terms stay below
-// 2.3e-17 of the sum; s comes from the hardware reciprocal with two Newton
+// 2.4e-17 of the sum; s comes from the hardware reciprocal with two Newton
 // steps and one residual correction, and e ln 2 is added in two parts (ln 2
 // rounded to 42 bits, so e * LN2_HI is exact, and the remainder).  About
 // 2 ulp for positive finite x (40 instructions; the library's log spends 98
