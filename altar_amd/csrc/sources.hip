@@ -182,8 +182,9 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_kernel(ObsView obs, const cdm::G
     // reach the arithmetic as vector registers (LDS broadcast reads): on
     // gfx950, back-to-back VALU instructions that read a scalar register
     // issue at about half the rate of all-VGPR ones (a rate probe: 4.1 against
-    // 2.2 cycles per wave64 multiply).  The FP32 pass at 16,384 chains x
-    // 5,000 LOS went from 4.06 to 3.67 ms with the same results bit for bit.
+    // 2.2 cycles per wave64 multiply).  With the constants read by scalar
+    // loads instead, the FP32 pass at 16,384 chains x 5,000 LOS takes 10% longer
+    // (same results bit for bit).
     __shared__ cdm::Geom<float> sG;
     {
         static_assert(sizeof(cdm::Geom<float>) % 16 == 0, "FP32 geometry is copied in 16-byte pieces");
@@ -351,7 +352,9 @@ __global__ __launch_bounds__(OBS_TILE) void cdm_fp64_kernel(ObsView obs, const c
         if ((fl & (cdm::FLAG_INVALID | cdm::FLAG_NAN)) || !((fl >> 12) & ~fl & 0xfffu)) continue;
         __shared__ cdm::Geom<double> sH;
         {
+            static_assert(sizeof(cdm::Geom<double>) % 16 == 0, "FP64 geometry is copied in 16-byte pieces");
             constexpr int NV4 = (int)(sizeof(cdm::Geom<double>) / 16);
+            static_assert(NV4 <= OBS_TILE, "one piece per thread");
             const float4 *src = reinterpret_cast<const float4 *>(g64 + chain);
             __syncthreads();
             if (threadIdx.x < NV4) reinterpret_cast<float4 *>(&sH)[threadIdx.x] = src[threadIdx.x];
