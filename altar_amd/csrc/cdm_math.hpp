@@ -191,7 +191,7 @@ ALTAR_HD inline double side_angle(double y, double x, bool exact) { return exact
 template <typename T>
 struct Side {
     T e1x, e1y;          // first ADCS axis (horizontal); ey2 = (e1y, -e1x, 0), ey3 = -ez
-    T y1AB;              // ADCS y1 of PB - PA (libcdm.py:337-339); its y2 is zero, since ey2 is
+    T y1AB;              // ADCS y1 of PB - PA (libcdm.py:335-337); its y2 is zero, since ey2 is
                          // normal to the side's horizontal direction, so both vertices share y2
     T dz1, dz3;          // z1B - z1A and z3B - z3A (configuration II; configuration I flips both)
     T b1, b2, b3;        // Burgers vector in ADCS, divided by 2 pi
