@@ -58,7 +58,7 @@ def build(verbose=False, jobs=4):
     tmp = LIB + ".tmp"
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + \
-        ["-L", os.path.join(rocm, "lib"), "-lrocblas", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+        ["-L", os.path.join(rocm, "lib"), "-Wl,-rpath," + os.path.join(rocm, "lib")]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -7,7 +7,6 @@
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -129,9 +128,7 @@ struct altar_source {
     // workspace, sized for Ccap chains (Cpad rows)
     int Ccap = 0;
     size_t Cpad = 0;
-    DevBuf R32, Rhi, Rlo, R64, Y64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census;
-    // rocBLAS handle of the FP64 whitening (dtrmm), created on first use
-    rocblas_handle blas = nullptr;
+    DevBuf R32, Rhi, Rlo, R64, partial, T32, Thi, Tlo, prm, g32, g64, flags, shifts, census;
     // optional per-kernel timing with HIP events on the launch stream
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks[ALTAR_TIMER_SLOTS];
@@ -158,7 +155,7 @@ struct altar_source {
         Ccap = C;
         Cpad = round_up((size_t)C, 128);
         // zero-initialised: the GEMMs read the padding columns
-        R32.release(); Rhi.release(); Rlo.release(); R64.release(); Y64.release(); T32.release(); Thi.release();
+        R32.release(); Rhi.release(); Rlo.release(); R64.release(); T32.release(); Thi.release();
         Tlo.release();
         partial.release(); prm.release(); g32.release(); g64.release(); flags.release(); shifts.release(); census.release();
     }
@@ -211,8 +208,6 @@ int altar_source_destroy(altar_source *h)
                 (void)hipEventDestroy(p.second);
             }
         h->census.release();
-        h->Y64.release();
-        if (h->blas) (void)rocblas_destroy_handle(h->blas);
         for (WBlock &b : h->blocks) {
             b.W32.release(); b.Whi.release(); b.Wlo.release();
         }
@@ -338,9 +333,9 @@ int altar_source_whitening(altar_source *h, int kind, const double *W, double no
                 wl[i * Np + j] = to_bf16(v - from_bf16(hi));
             }
         upload(h->W32, w); upload(h->Whi, wh); upload(h->Wlo, wl);
-        std::vector<double> w64(N * N, 0.0);
+        std::vector<double> w64(Np * Np, 0.0);
         for (size_t i = 0; i < N; ++i)
-            for (size_t j = 0; j <= i; ++j) w64[i * N + j] = W[i * N + j];
+            for (size_t j = 0; j <= i; ++j) w64[i * Np + j] = W[i * N + j];
         upload(h->W64, w64);
     });
 }
@@ -827,35 +822,26 @@ void data_llk_impl(altar_source *h, const double *theta_dev, int C, double *llk_
         const bool blocked = h->wkind == ALTAR_WHITEN_BLOCKS;
         if (precision == ALTAR_FP64) {
             // the reference's precision: FP64 residuals (Mogi in FP64; CDM
-            // with every side in FP64), W r by rocBLAS dtrmm, FP64 sums
+            // with every side in FP64), |W r|^2 by the FP64 MFMA GEMM with
+            // its sum-of-squares epilogue, FP64 sums
             require(!blocked && (h->model == ALTAR_MOGI || h->model == ALTAR_CDM),
                     "FP64 full whitening is offered for Mogi and CDM sources with a lower-triangular whitener");
             require(h->model != ALTAR_CDM || h->fp64_sin > 1.0,
                     "CDM FP64 llk: evaluate every side in FP64 first (altar_source_fp64_sides(h, 2))");
-            const size_t N = (size_t)h->N;
-            h->R64.reserve(Cp * N * sizeof(double));
-            h->Y64.reserve(Cp * N * sizeof(double));
+            // padding columns stay zero (the source writes columns below N only)
+            h->R64.reserve(Cp * Np * sizeof(double), true);
             Out o{};
-            o.ld = N;
+            o.ld = Np;
             o.r64 = h->R64.as<double>();
             run_source(h, theta_dev, C, MODE_RES_F64, h->model == ALTAR_MOGI, false, o, st);
-            if (!h->blas) {
-                rocblas_handle b = nullptr;
-                require(rocblas_create_handle(&b) == rocblas_status_success, "rocblas_create_handle failed");
-                h->blas = b;
+            h->partial.reserve(Cp * ntn * sizeof(double));
+            {
+                Mark m(h, 1, st);
+                hip_check(launch_gemm_f64_sumsq(h->R64.as<double>(), Np, h->W64.as<double>(), Np, (int)Mp, (int)Np,
+                                                (int)Np, 1, h->N, h->partial.as<double>(), ntn, st),
+                          "FP64 whitening");
             }
-            require(rocblas_set_stream(h->blas, st) == rocblas_status_success, "rocblas_set_stream failed");
-            // row-major R (C x N) is column-major X = R^T (N x C); row-major
-            // lower W is column-major upper W^T; Y^T = W X = op(W^T) X with op = T
-            const double one = 1.0;
-            Mark m(h, 1, st);
-            require(rocblas_dtrmm(h->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                                  rocblas_diagonal_non_unit, (rocblas_int)N, (rocblas_int)C, &one,
-                                  h->W64.as<double>(), (rocblas_int)N, h->R64.as<double>(), (rocblas_int)N,
-                                  h->Y64.as<double>(), (rocblas_int)N) == rocblas_status_success,
-                    "rocblas_dtrmm failed");
-            hip_check(launch_sumsq_rows_f64(h->Y64.as<double>(), N, (int)N, C, h->normalization, llk_dev, st),
-                      "FP64 sums of squares");
+            hip_check(launch_llk_reduce(h->partial.p, 1, ntn, C, h->normalization, llk_dev, st), "reduce");
             return;
         }
         int ntiles = ntn;

@@ -13,7 +13,9 @@
 // FP32 path: v_mfma_f32_32x32x2_f32.  Split-BF16 path: operands are stored
 // as hi + lo BF16 pairs and each tile accumulates hi*hi + hi*lo + lo*hi with
 // v_mfma_f32_32x32x16_bf16 in FP32 — one BF16 product plus one correction
-// (refinement) step; with plain=1 only hi*hi is formed.
+// (refinement) step; with plain=1 only hi*hi is formed.  FP64 path (the
+// reference's precision, whitening only): v_mfma_f64_16x16x4_f64 with the
+// sum-of-squares epilogue in FP64.
 //
 // Tiling: 256 threads = 4 wavefronts in a 2 x 2 grid; block tile 128 x 128;
 // each wavefront owns 64 x 64 = 2 x 2 MFMA 32x32 tiles.  Operands are
@@ -355,6 +357,138 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3(const __bf16 *__restrict__
     gemm_epilogue<EPI>(acc, ep, m0, n0, tn, wm, wn, red);
 }
 
+// ---------------------------------------------------------------- FP64
+// The reference-precision whitening: partial[m, tile] = sum over the tile's
+// 128 columns n of (sum_k A[m, k] B[n, k])^2, every product and sum in FP64
+// (v_mfma_f64_16x16x4_f64).  Block tile and wavefront grid as in the FP32
+// path; each wavefront owns 64 x 64 = 4 x 4 MFMA 16x16 tiles (4 FP64
+// accumulators per lane each).  A K step is 8 doubles = two MFMAs: the lanes
+// of group g = lane / 16 read the 16-byte chunk g of their row, k = 2g and
+// 2g + 1, and feed k = 2g to the first MFMA and 2g + 1 to the second (a
+// permutation of the summation order, the same for A and B).  Operand
+// layout of the FP64 16x16x4 MFMA: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator register v of lane l holds row l / 16 + 4 v,
+// column l % 16 (rows interleaved, unlike the FP32 16x16 layout; measured).  Two waves per SIMD: the 64 FP64 accumulators per lane
+// (128 registers) and the fragments fit 256 registers without spilling.
+constexpr int BK64 = 8;
+constexpr int LDS64 = BK64;  // row pitch (doubles): 64 bytes, chunk-swizzled like the FP32 tiles
+
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_nt_f64_sumsq(const double *__restrict__ A, size_t lda,
+                                                         const double *__restrict__ B, size_t ldb, int Kp,
+                                                         int ntn, int tri, int nvalid,
+                                                         double *__restrict__ partial, int ntiles)
+{
+    __shared__ __attribute__((aligned(16))) double As[2][BM * LDS64];
+    __shared__ __attribute__((aligned(16))) double Bs[2][BN * LDS64];
+    __shared__ double red[2 * BM];
+
+    int tn, tm;
+    tile_coords(tri, ntn, tn, tm);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int kend = tri ? min(Kp, n0 + BN) : Kp;
+    const int nk = kend / BK64;
+
+    // global -> register staging: thread t moves chunk t % 4 (doubles
+    // 2 (t % 4), +1) of rows t/4 and t/4 + 64 of each operand tile
+    const int lr = tid >> 2, lc = (tid & 3) * 2;
+    const int ls = 2 * ((tid & 3) ^ swz(lr));  // swz(lr + 64) == swz(lr)
+    const double *Ag0 = A + (size_t)(m0 + lr) * lda + lc;
+    const double *Ag1 = A + (size_t)(m0 + lr + 64) * lda + lc;
+    const double *Bg0 = B + (size_t)(n0 + lr) * ldb + lc;
+    const double *Bg1 = B + (size_t)(n0 + lr + 64) * ldb + lc;
+
+    doublex2 ra0, ra1, rb0, rb1;
+    auto gload = [&](int kt) {
+        const int k = kt * BK64;
+        ra0 = *(const doublex2 *)(Ag0 + k);
+        ra1 = *(const doublex2 *)(Ag1 + k);
+        rb0 = *(const doublex2 *)(Bg0 + k);
+        rb1 = *(const doublex2 *)(Bg1 + k);
+    };
+    auto sstore = [&](int buf) {
+        *(doublex2 *)(&As[buf][lr * LDS64 + ls]) = ra0;
+        *(doublex2 *)(&As[buf][(lr + 64) * LDS64 + ls]) = ra1;
+        *(doublex2 *)(&Bs[buf][lr * LDS64 + ls]) = rb0;
+        *(doublex2 *)(&Bs[buf][(lr + 64) * LDS64 + ls]) = rb1;
+    };
+
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+    const int g = lane >> 4, l16 = lane & 15;
+    // fragment rows are 16-aligned bases + l16: the swizzle depends on l16 only
+    const int cg = 2 * (g ^ swz(l16));
+    if (nk > 0) {
+        gload(0);
+        sstore(0);
+    }
+    __syncthreads();
+    // NI: how many of this wave's four 16-column MFMA tiles hold a column
+    // below nvalid (the others multiply zero rows of B and are skipped)
+    auto kloop = [&](auto NI) {
+        for (int kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            if (kt + 1 < nk) gload(kt + 1);
+            doublex2 a[4], b[4];
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+                a[mi] = *(const doublex2 *)(&As[cur][(wm * 64 + mi * 16 + l16) * LDS64 + cg]);
+#pragma unroll
+            for (int ni = 0; ni < decltype(NI)::value; ++ni)
+                b[ni] = *(const doublex2 *)(&Bs[cur][(wn * 64 + ni * 16 + l16) * LDS64 + cg]);
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < decltype(NI)::value; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi][kk], b[ni][kk], acc[mi][ni], 0, 0, 0);
+            if (kt + 1 < nk) sstore(cur ^ 1);
+            __syncthreads();
+        }
+    };
+    const int col0 = n0 + wn * 64;
+    const int nt = nvalid <= 0 ? 4 : max(0, min(4, (nvalid - col0 + 15) / 16));
+    switch (nt) {
+    case 4: kloop(std::integral_constant<int, 4>{}); break;
+    case 3: kloop(std::integral_constant<int, 3>{}); break;
+    case 2: kloop(std::integral_constant<int, 2>{}); break;
+    case 1: kloop(std::integral_constant<int, 1>{}); break;
+    default: kloop(std::integral_constant<int, 0>{}); break;
+    }
+
+    // sum of squares over this block's 128 columns, per row (the skipped
+    // tiles' accumulators are zero)
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) s[v] += acc[mi][ni][v] * acc[mi][ni][v];
+        // the 16 lanes of a group share rows: reduce over l16
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) s[v] += __shfl_xor(s[v], off, 64);
+        }
+        if (l16 == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[wn * BM + wm * 64 + mi * 16 + g + 4 * v] = s[v];
+        }
+    }
+    __syncthreads();
+    if (tid < BM) partial[(size_t)(m0 + tid) * ntiles + tn] = red[tid] + red[BM + tid];
+}
+
 // ---------------------------------------------------------------- reductions
 // llk[c] = normalization - sum_t partial[c, t] / 2, summed in FP64 in a fixed
 // order (deterministic).
@@ -368,23 +502,6 @@ __global__ void llk_reduce_kernel(const T *__restrict__ partial, int ntiles, int
     double s = 0.0;
     for (int t = 0; t < ntiles; ++t) s += (double)p[t];
     llk[c] = normalization - 0.5 * s;
-}
-
-// llk[c] = normalization - |y_c|^2 / 2 for the FP64 rows y_c (N values, row
-// pitch ld): one block per chain, each thread a strided partial sum, the
-// wave sums then the four waves in order (deterministic).
-__global__ __launch_bounds__(256) void sumsq_rows_f64_kernel(const double *__restrict__ Y, size_t ld, int N,
-                                                             double normalization, double *__restrict__ llk)
-{
-    const int c = blockIdx.x;
-    const double *y = Y + (size_t)c * ld;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) acc += y[i] * y[i];
-    __shared__ double red[4];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) llk[c] = normalization - 0.5 * (((red[0] + red[1]) + red[2]) + red[3]);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -420,6 +537,18 @@ hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, cons
         hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
     else
         hipLaunchKernelGGL((gemm_nt_bf16x3<EPI_SUMSQ_BIAS>), grid, block, 0, st, ah, al, lda, bh, bl, ldb, Kp, ntn, tri, plain, ep);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_f64_sumsq(const double *A, size_t lda, const double *B, size_t ldb, int Mp, int Np, int Kp,
+                                 int tri, int nvalid, double *partial, int ntiles, hipStream_t st)
+{
+    if (Mp % BM || Np % BN || Kp % BK64 || lda < (size_t)Kp || ldb < (size_t)Kp || lda % 2 || ldb % 2)
+        return hipErrorInvalidValue;
+    const int ntn = Np / BN;
+    if (ntiles < ntn) return hipErrorInvalidValue;
+    dim3 grid(ntn * (Mp / BM)), block(256);
+    hipLaunchKernelGGL(gemm_nt_f64_sumsq, grid, block, 0, st, A, lda, B, ldb, Kp, ntn, tri, nvalid, partial, ntiles);
     return hipGetLastError();
 }
 
@@ -480,13 +609,6 @@ hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, d
     else
         hipLaunchKernelGGL((llk_reduce_kernel<float>), grid, block, 0, st, (const float *)partial, ntiles, C,
                            normalization, llk);
-    return hipGetLastError();
-}
-
-hipError_t launch_sumsq_rows_f64(const double *Y, size_t ld, int N, int C, double normalization, double *llk,
-                                 hipStream_t st)
-{
-    hipLaunchKernelGGL(sumsq_rows_f64_kernel, dim3(C), dim3(256), 0, st, Y, ld, N, normalization, llk);
     return hipGetLastError();
 }
 

@@ -142,15 +142,16 @@ hipError_t launch_gemm_f32(const float *A, size_t lda, const float *B, size_t ld
 hipError_t launch_gemm_bf16x3(const void *Ahi, const void *Alo, size_t lda, const void *Bhi, const void *Blo,
                               size_t ldb, int Mp, int Np, int Kp, int tri, int plain, int epi,
                               const GemmEpi &ep, hipStream_t st);
+// partial[m, t] (FP64, Mp x ntiles) = |row m of A B^T over column tile t|^2,
+// all in FP64; operands zero-padded like the FP32 path's
+hipError_t launch_gemm_f64_sumsq(const double *A, size_t lda, const double *B, size_t ldb, int Mp, int Np, int Kp,
+                                 int tri, int nvalid, double *partial, int ntiles, hipStream_t st);
 hipError_t launch_gather_rows_f32(const float *src, size_t lds, const int *idx, int n, int cols, float *dst,
                                   size_t ldd, hipStream_t st);
 hipError_t launch_transpose_f32(const float *src, size_t lds, int rows, int cols, float *dst, size_t ldd,
                                 hipStream_t st);
 hipError_t launch_llk_reduce(const void *partial, int fp64, int ntiles, int C, double normalization,
                              double *llk, hipStream_t st);
-// llk[c] = normalization - |Y[c, 0:N]|^2 / 2 in FP64 (one block per chain)
-hipError_t launch_sumsq_rows_f64(const double *Y, size_t ld, int N, int C, double normalization, double *llk,
-                                 hipStream_t st);
 
 // sampler.hip
 hipError_t launch_normals(double *xi, int C, int P, uint64_t seed, uint32_t stage, uint32_t step, uint32_t tag,

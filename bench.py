@@ -393,7 +393,7 @@ def cdm_variant_timing(annealer, model, step, precision, fp64_sides, steps):
             "cdm_forward_tflops": C * N * CDM_FLOPS_PER_PAIR / (fwd * 1e-3) / 1e12,
             "whitening_ms": gemm, "whitening_tflops_algorithmic": C * N * (N + 1) / (gemm * 1e-3) / 1e12,
             "whitening": {"fp32": "gemm_nt_f32 (MFMA)", "bf16r": "gemm_nt_bf16x3 (MFMA)",
-                          "fp64": "rocBLAS dtrmm"}.get(precision),
+                          "fp64": "gemm_nt_f64_sumsq (FP64 MFMA)"}.get(precision),
             "fp64_sides_per_chain": f64s / max(C - invalid, 1)}
 
 

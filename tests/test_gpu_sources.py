@@ -94,15 +94,18 @@ def test_mogi_llk_diag(cuda, oracle):
         assert rel.max() < tol, prec
 
 
-@pytest.mark.parametrize("N", [300, 1000])
-def test_mogi_llk_full_cd(cuda, oracle, N):
-    print(f"case: mogi llk full Cd C=512 N={N}", flush=True)
+# N = 300, 1,000, 1,100 and 1,120 leave 3 + 0, 4 + 3, 4 + 1 and 4 + 2 of the
+# last column tile's 16-column FP64 MFMA tiles with valid columns (the rest
+# are skipped); C = 333 and 200 leave part of the last 128-row tile unused
+@pytest.mark.parametrize("N,C", [(300, 512), (1000, 512), (1100, 333), (1120, 200)])
+def test_mogi_llk_full_cd(cuda, oracle, N, C):
+    print(f"case: mogi llk full Cd C={C} N={N}", flush=True)
     case = mogi_case(N=N)
     rng = np.random.default_rng(3)
     xy = np.stack([case["x"], case["y"]], 1)
     cd = norms.exponential_covariance(xy, 25e-6, 5e3)
     case["d"] = case["d"] + np.linalg.cholesky(cd) @ rng.standard_normal(N)
-    theta = mogi_theta(rng, 512)
+    theta = mogi_theta(rng, C)
     theta[:16] = case["truth"] + rng.normal(0, [50, 50, 50, 0.01, 1e-3], (16, 5))
     res = oracle.mogi_residuals(theta, (0, 2, 3, 4), case["x"], case["y"], case["los"], case["d"],
                                 case["oid"], 0.25)
@@ -120,7 +123,7 @@ def test_mogi_llk_full_cd(cuda, oracle, N):
         src.whitening(kind, W, norm)
         # measured (both modes, N = 300 and 1,000): fp64 4.7e-14, fp32 7.8e-6, bf16r 1.5e-4
         for prec, tol in (("fp64", 2e-13), ("fp32", 3e-5), ("bf16r", 5e-4), ("bf16", None)):
-            out = torch.empty(512, dtype=torch.float64, device=cuda)
+            out = torch.empty(C, dtype=torch.float64, device=cuda)
             src.data_llk(t, out, prec)
             got = out.cpu().numpy()
             rel = np.abs(got - ref) / np.abs(ref)
@@ -375,8 +378,8 @@ def test_fields_diagonal_and_prewhitened(cuda, oracle):
 
 def test_cdm_llk_fp64_reference_precision(cuda, oracle):
     """The reference's precision end to end: every CDM side in FP64 (FP64
-    atan2), the residual formed in FP64, W r by rocBLAS dtrmm in FP64 and FP64
-    sums of squares, against the oracle's FP64 llk.  With FP32 sides the FP64
+    atan2), the residual formed in FP64, |W r|^2 by the FP64 MFMA whitening
+    GEMM with FP64 sums, against the oracle's FP64 llk.  With FP32 sides the FP64
     llk is refused (the sides would not be reference precision)."""
     print("case: cdm llk FP64 reference precision C=64 N=1200", flush=True)
     rng = np.random.default_rng(16)
