@@ -178,10 +178,74 @@ ALTAR_HD inline float f_atan2(float y, float x)
 // random sources against the FP64 oracle); the logarithms and reciprocals,
 // whose coefficients grow like cot(beta)^2, stay in FP64.
 ALTAR_HD inline double f_atan2(double y, double x) { return (double)f_atan2((float)y, (float)x); }
+
+// FP64 quotient a / b: the Newton-refined reciprocal and one residual
+// correction of the product
+ALTAR_HD inline double f_div_refined(double a, double b)
+{
+    const double r = f_rcp(b);
+    const double q = a * r;
+    return fma(fma(-q, b, a), r, q);
+}
+
+// FP64 atan2 of the every-side-FP64 mode (the reference's precision).  On
+// the device: the octant reduction of f_atan2 to t = min / max in [0, 1];
+// above tan(pi/8), atan t = atan c + atan u with u = (t - c) / (1 + c t),
+// c = 1/2 up to t = 11/16 and c = 1 beyond (t - c is exact there), so that
+// |u| <= tan(pi/8) and atan u = u + u s P(s), s = u^2, with P of degree 10
+// (interpolation of (atan(sqrt s) / sqrt s - 1) / s at the 11 Chebyshev
+// nodes of [0, tan(pi/8)^2] in 60-digit arithmetic, coefficients rounded to
+// FP64: 7e-18 relative); the quotients by f_div_refined; atan c, pi/2 and pi
+// added in two parts, with the rounding error of the leading sum kept.
+// At most 1.06 ulp on 3,000 test points spread over the circle and the
+// reduction boundaries (finite arguments).  The sign of y,
+// including -0, selects the branch, and atan2(0, 0) = 0.  Host builds use
+// atan2().
+ALTAR_HD inline double f_atan2_exact(double y, double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr double PIO4_HI = 0x1.921fb54442d18p-1, PIO4_LO = 0x1.1a62633145c07p-55;
+    constexpr double PIO2_HI = 0x1.921fb54442d18p+0, PIO2_LO = 0x1.1a62633145c07p-54;
+    constexpr double PI_HI = 0x1.921fb54442d18p+1, PI_LO = 0x1.1a62633145c07p-53;
+    constexpr double ATAN05_HI = 0x1.dac670561bb4fp-2, ATAN05_LO = 0x1.a2b7f222f65e2p-56;
+    constexpr double TAN_PIO8 = 0x1.a827999fcef32p-2;
+    const double ax = fabs(x), ay = fabs(y);
+    const double mx = fmax(ax, ay), mn = fmin(ax, ay);
+    const double t = mx > 0.0 ? f_div_refined(mn, mx) : 0.0;
+    const bool red = t > TAN_PIO8, one = t > 0x1.6p-1;
+    const double c = one ? 1.0 : 0.5;
+    const double u = red ? f_div_refined(t - c, fma(c, t, 1.0)) : t;
+    const double s = u * u;
+    double p = -0x1.3a31b1c0fd3b7p-6;
+    p = fma(p, s, 0x1.4162c02b1dda3p-5);
+    p = fma(p, s, -0x1.a0999c632b6edp-5);
+    p = fma(p, s, 0x1.dfe6497e96323p-5);
+    p = fma(p, s, -0x1.10fa77b1a6d57p-4);
+    p = fma(p, s, 0x1.3b1263064f6b9p-4);
+    p = fma(p, s, -0x1.745d0b28a7e37p-4);
+    p = fma(p, s, 0x1.c71c71853d7fap-4);
+    p = fma(p, s, -0x1.2492492436201p-3);
+    p = fma(p, s, 0x1.999999999934cp-3);
+    p = fma(p, s, -0x1.5555555555555p-2);
+    double a = fma(u * s, p, u);
+    // h + a + l with |a| <= |h|: the rounding error of h + a recovered exactly
+    // (Dekker's fast two-sum) and added with the constant's low part
+    auto add3 = [](double h, double a, double l) {
+        const double sum = h + a;
+        return sum + (((h - sum) + a) + l);
+    };
+    a = red ? add3(one ? PIO4_HI : ATAN05_HI, a, one ? PIO4_LO : ATAN05_LO) : a;
+    a = ay > ax ? add3(PIO2_HI, -a, PIO2_LO) : a;
+    a = x < 0.0 ? add3(PI_HI, -a, PI_LO) : a;
+    return copysign(a, y);
+#else
+    return atan2(y, x);
+#endif
+}
 // The Burgers angle difference of a side: f_atan2, or with exact set (every
-// side in FP64, the reference's precision) the FP64 atan2.
+// side in FP64, the reference's precision) f_atan2_exact.
 ALTAR_HD inline float side_angle(float y, float x, bool) { return f_atan2(y, x); }
-ALTAR_HD inline double side_angle(double y, double x, bool exact) { return exact ? atan2(y, x) : f_atan2(y, x); }
+ALTAR_HD inline double side_angle(double y, double x, bool exact) { return exact ? f_atan2_exact(y, x) : f_atan2(y, x); }
 
 // Side constants of one angular-dislocation pair.  Configuration II uses
 // the angle beta and configuration I the angle beta - pi (libcdm.py:344-362):
